@@ -1,0 +1,195 @@
+"""Throughput of the p2z propagate-to-z + Kalman-update path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--math fast|ref|fp64gain]
+
+Workload: the reference's full configuration, 100 events x 9600 tracks x 20
+layers, AoSoA bsize 32, FP32, inputs from the reference generator (preset
+'ref': src/input_track.h, srand(1), smear 1e-7).  One step is one pass of the
+reference driver loop (src/propagate-toz-test_OMP.cpp:733-747) over all
+events: a single fused launch with tracks, hits and output resident in HBM.
+
+Prints one JSON line: value = track-layer updates/s, a roofline object for
+the fused kernel (algorithmic HBM bytes per launch / hipEvent launch time) and
+the CPU baseline: the reference program compiled from its own source
+(oracle/_ref, OpenMP over events) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "p2z-tests_amd"))
+
+import numpy as np  # noqa: E402
+
+import p2z  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+PROFILE_TRAFFIC = os.path.join(HERE, "profiles", "traffic_fused_w32_100ev.txt")
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def bytes_per_track_layer(nlayer: int) -> float:
+    """Algorithmic HBM bytes per track-layer: the 6 hit words the path reads
+    (x, y, z, cov xx, xy, yy) plus the 28-word track read once and written
+    once, spread over the layers."""
+    return 24.0 + 224.0 / nlayer
+
+
+def read_traffic(math: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), if the committed
+    profile summary for this workload and math mode exists."""
+    if not os.path.exists(PROFILE_TRAFFIC):
+        return None
+    with open(PROFILE_TRAFFIC) as f:
+        for ln in f:
+            m = re.match(rf"traffic_bytes_per_launch\[{math}\]\s*=\s*([0-9.eE+]+)", ln.strip())
+            if m:
+                return float(m.group(1))
+    return None
+
+
+def cpu_baseline(nevts: int, ntrks: int, nlayer: int) -> dict | None:
+    """The reference program (src/propagate-toz-test_OMP.cpp, built by
+    oracle/Makefile from the reference's own source with its flags) on the
+    same workload with NITER=2 and 16 OpenMP threads; its own timer brackets
+    the NITER region only (OMP:729-753)."""
+    cpuflags = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
+    has512 = all(f in cpuflags for f in ("avx512f", "avx512bw", "avx512dq", "avx512vl"))
+    exe = os.path.join(HERE, "oracle", "_ref", "ref_bench_v4" if has512 else "ref_bench_v3")
+    if not os.path.exists(exe):
+        return cpu_baseline_port(nevts, ntrks, nlayer)
+    log(f"cpu baseline: {os.path.basename(exe)}")
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    if res.returncode != 0:
+        log(res.stderr)
+        return None
+    m = re.search(r"done ntracks=(\d+) tot time=([0-9.]+) \(s\)", res.stdout)
+    f = re.search(r"formatted (\d+) (\d+) (\d+) (\d+) (\d+) [0-9.]+ 0 [0-9.]+ (\d+)", res.stdout)
+    ntr, secs = int(m.group(1)), float(m.group(2))
+    niter, ne, nt, threads = int(f.group(1)), int(f.group(2)), int(f.group(3)), int(f.group(6))
+    assert (ne, nt) == (nevts, ntrks), (ne, nt)
+    return {"value": ntr * nlayer / secs, "unit": "track-layer updates/s",
+            "cores": min(threads, ne), "kind": "reference",
+            "sample": f"reference OpenMP program built from its source ({os.path.basename(exe)}), "
+                      f"{ne} events x {nt} tracks x {nlayer} layers, NITER={niter}, "
+                      f"{threads} OpenMP threads over events, {secs:.3f} s timed"}
+
+
+def cpu_baseline_port(nevts: int, ntrks: int, nlayer: int) -> dict | None:
+    """Fallback when the reference build is absent: the oracle restatement
+    with the reference's threading (parallel over events), 2 passes."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    try:
+        import pyoracle
+        pyoracle.lib()
+    except (ImportError, FileNotFoundError):
+        return None
+    W, threads = 32, 16
+    trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nlayer)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        pyoracle.run_events_only(trk, hit, nevts, nb, W, nlayer, threads)
+    secs = time.perf_counter() - t0
+    return {"value": 2 * nevts * nb * W * nlayer / secs, "unit": "track-layer updates/s",
+            "cores": min(threads, nevts), "kind": "port",
+            "sample": f"oracle restatement, {nevts} events x {ntrks} tracks x {nlayer} layers, 2 passes, "
+                      f"{threads} threads over events, {secs:.3f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--math", default="fast", choices=list(p2z.MATH))
+    ap.add_argument("--nevts", type=int, default=100)
+    ap.add_argument("--ntrks", type=int, default=9600)
+    ap.add_argument("--nlayer", type=int, default=20)
+    ap.add_argument("--bsize", type=int, default=32)
+    ap.add_argument("--tracks-per-wg", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this path shards by event with no exchange step, and "
+              "this build runs on one GPU only (--gpus 1).", file=sys.stderr)
+        return 2
+
+    nevts, ntrks, nl, W = a.nevts, a.ntrks, a.nlayer, a.bsize
+    log(f"generating {nevts} events x {ntrks} tracks x {nl} layers (reference generator)")
+    trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
+    ctx = p2z.Context(0, tracks_per_wg=a.tracks_per_wg)
+    dt, dh, do = ctx.upload(trk), ctx.upload(hit), ctx.alloc(trk.nbytes)
+    tl_per_step = nevts * nb * W * nl
+
+    log(f"warmup {a.warmup} steps")
+    for _ in range(a.warmup):
+        ctx.run(dt, dh, do, nevts, nb, nl, W, a.math)
+    ctx.sync()
+    log(f"timing {a.steps} steps")
+    wall0 = time.perf_counter()
+    ctx.timer_start()
+    for _ in range(a.steps):
+        ctx.run(dt, dh, do, nevts, nb, nl, W, a.math)
+    ms = ctx.timer_stop()
+    ctx.sync()
+    wall = time.perf_counter() - wall0
+    ms_per_step = ms / a.steps
+    value = tl_per_step / (ms_per_step * 1e-3)
+
+    # sanity on the result (product only: no oracle here): finite, z on the last hit plane
+    out = ctx.download(do)
+    z_ok = bool(np.all(out.reshape(nevts * nb, 28, W)[:, 2, :] == hit.reshape(nevts * nb, nl, 9, W)[:, nl - 1, 2, :]))
+    ctx.close()
+
+    b_tl = bytes_per_track_layer(nl)
+    achieved = tl_per_step * b_tl / (ms_per_step * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(a.math),
+                "bytes_per_track_layer": b_tl, "launch_ms": round(ms_per_step, 5)}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline(nevts, ntrks, nl)
+
+    rec = {
+        "metric": "track-layer updates/s on 1 MI355X",
+        "value": value,
+        "unit": "track-layer updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if a.math != "fp64gain" else "f32+f64gain",
+        "data": "synthetic: reference generator (src/input_track.h, srand(1), smear 1e-7)",
+        "config": {"workload": f"propagateToZ+KalmanUpdate_v2 driver pass, {nevts} events x {ntrks} tracks x "
+                               f"{nl} layers, AoSoA bsize {W}, fused single launch, inputs resident in HBM",
+                   "nevts": nevts, "ntrks": ntrks, "nlayer": nl, "bsize": W, "math": a.math,
+                   "parallelism": "one GPU, one track per lane"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "wall_s_timed": round(wall, 4),
+        "z_on_last_hit_plane": z_ok,
+    }
+    if cpu:
+        rec["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(rec), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,154 @@
+/*
+ * p2z -- MI355X (gfx950) propagate-to-z + Kalman-update path, C ABI.
+ *
+ * Drop-in for the hot path of cerati/p2z-tests: the per-block functions
+ * propagateToZ / KalmanUpdate_v2 and the driver loop of
+ * src/propagate-toz-test_OMP.cpp (cited as OMP:<line> below), with sizes as
+ * runtime arguments instead of the reference's compile-time macros.
+ *
+ * Data layout is the reference's, unchanged (zero-copy):
+ *   MPTRK block (OMP:115-120): par[6W] | cov[21W] | q[W] (int32), 28W words
+ *   MPHIT block (OMP:122-125): pos[3W] | cov[6W],                  9W words
+ *   element e of lane l of a block is word e*W + l (OMP:159-161)
+ *   track block of (event ie, block ib): ib + nb*ie          (bTk, OMP:147)
+ *   hit block of (ie, ib, layer lay):   lay + nlayer*(ib+nb*ie) (bHit, OMP:215)
+ * W (the reference's bsize) is one of 16, 32, 64, 128.
+ *
+ * Conventions: every function returns P2Z_OK (0) or a negative P2Z_E* code and
+ * never exits or throws; p2z_last_error() describes the last failure of the
+ * calling thread.  Functions taking "device" pointers enqueue work on the
+ * context's stream and return before it finishes (p2z_sync waits).  A context
+ * is used by one host thread at a time; distinct contexts are independent.
+ */
+#ifndef P2Z_H
+#define P2Z_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define P2Z_ABI_VERSION 1
+
+/* status codes */
+#define P2Z_OK 0
+#define P2Z_EINVAL (-1)   /* bad argument (size, width, pointer, mode)      */
+#define P2Z_ENOMEM (-2)   /* device or host allocation failed, or budget    */
+#define P2Z_EHIP (-3)     /* HIP runtime error (message in p2z_last_error)  */
+#define P2Z_ENODEV (-4)   /* no such GPU                                    */
+#define P2Z_EBUSY (-5)    /* serving: no free slot / budget right now       */
+#define P2Z_ESTATE (-6)   /* call not valid in the object's current state   */
+
+/* arithmetic modes (see DESIGN.md of the repository for the numerical contract) */
+#define P2Z_MATH_FAST 0     /* FP32, FMA contraction, sparse forms (default)  */
+#define P2Z_MATH_REF 1      /* the reference CPU typing: FP64 sub-expressions,
+                               no contraction, reference summation order      */
+#define P2Z_MATH_FP64GAIN 2 /* FP32 propagation; FP64 2x2 inverse, gain and
+                               Joseph-form covariance update                  */
+
+/* Input records (reference ATRK / AHIT, OMP:57-67). */
+typedef struct { float par[6]; float cov[21]; int32_t q; } p2z_atrk;
+typedef struct { float pos[3]; float cov[6]; } p2z_ahit;
+
+typedef struct p2z_ctx p2z_ctx;
+
+typedef struct {
+  int32_t tracks_per_wg; /* workgroup size in tracks (64..1024, multiple of 64); 0 = 256 */
+  int32_t nstreams;      /* streams of the host-buffer pipeline; 0 = 4            */
+  int32_t hit_stage;     /* LDS hit-staging depth in layers; 0 = register prefetch */
+  int32_t reserved;
+} p2z_opts;
+
+/* ---- library, context, memory ------------------------------------------ */
+int p2z_version(void);
+const char* p2z_last_error(void);
+int p2z_device_count(int* n);
+int p2z_create(p2z_ctx** ctx, int device, const p2z_opts* opts /* NULL = defaults */);
+int p2z_destroy(p2z_ctx* ctx);
+int p2z_sync(p2z_ctx* ctx);
+int p2z_set_opts(p2z_ctx* ctx, const p2z_opts* opts);
+int p2z_get_stream(p2z_ctx* ctx, void** hip_stream);
+int p2z_dev_alloc(p2z_ctx* ctx, void** dptr, size_t bytes);
+int p2z_dev_free(p2z_ctx* ctx, void* dptr);
+int p2z_host_alloc(p2z_ctx* ctx, void** hptr, size_t bytes); /* pinned */
+int p2z_host_free(p2z_ctx* ctx, void* hptr);
+int p2z_memcpy_h2d(p2z_ctx* ctx, void* dst, const void* src, size_t bytes); /* async */
+int p2z_memcpy_d2h(p2z_ctx* ctx, void* dst, const void* src, size_t bytes); /* async */
+/* hipEvent timer on the context's stream: start, then stop returns elapsed ms */
+int p2z_timer_start(p2z_ctx* ctx);
+int p2z_timer_stop(p2z_ctx* ctx, float* ms);
+
+/* ---- generator (host) ----------------------------------------------------
+ * Replaces randn / prepareTracks / prepareHits (OMP:127-145, 243-263, 265-293):
+ * same libc rand() stream, same draw order and typing.  p2z_srand resets the
+ * stream (srand) and the cached second variate. */
+void p2z_srand(unsigned seed);
+float p2z_randn(float mu, float sigma);
+/* out: nevts*nb MPTRK blocks */
+int p2z_prepare_tracks(const p2z_atrk* in, void* out, int64_t nevts, int64_t nb, int32_t width,
+                       double smear);
+/* out: nlayer*nevts*nb MPHIT blocks; layer lay uses hits[lay]; P2Z_EINVAL when
+ * nlayer > nhits (the reference calls exit(1), OMP:271-274). */
+int p2z_prepare_hits(const p2z_ahit* hits, int32_t nhits, void* out, int64_t nevts, int64_t nb,
+                     int32_t width, int32_t nlayer, double smear);
+
+/* Input presets of the drop-in program.  P2Z_PRESET_REF is the reference's
+ * own input (src/input_track.h:1-113 in main's layer order, OMP:693-695;
+ * smear 1e-7, OMP:30); P2Z_PRESET_CONFIG0 the nominal track and hit of
+ * propagate-toz-pseudocode.txt:2-15 (charge -1, same hit on every layer,
+ * smear 1e-5).  Fills trk and hits[0..25] (cap >= 26) and *smear; returns the
+ * number of hits (26) or P2Z_EINVAL. */
+#define P2Z_PRESET_REF 0
+#define P2Z_PRESET_CONFIG0 1
+int p2z_preset_input(int32_t preset, p2z_atrk* trk, p2z_ahit* hits, int32_t cap, double* smear);
+
+/* ---- the path (device pointers) -----------------------------------------
+ * propagateToZ (OMP:623-682) on nblocks blocks.  inErr/inPar/inChg/outErr/
+ * outPar point at block 0 of arrays whose consecutive blocks are trk_stride
+ * 32-bit words apart (28*W for an MPTRK array); msP likewise with hit_stride
+ * (9*W for an MPHIT array).  in and out may alias (the reference calls it in
+ * place, OMP:742). */
+int p2z_propagate_to_z(p2z_ctx* ctx, const float* inErr, const float* inPar, const int32_t* inChg,
+                       const float* msP, float* outErr, float* outPar, int64_t nblocks,
+                       int64_t trk_stride, int64_t hit_stride, int32_t width, int32_t math);
+/* KalmanUpdate_v2 (OMP:503-620), in place, same stride convention. */
+int p2z_kalman_update(p2z_ctx* ctx, float* trkErr, float* inPar, const float* hitErr,
+                      const float* msP, int64_t nblocks, int64_t trk_stride, int64_t hit_stride,
+                      int32_t width, int32_t math);
+/* The driver loop (OMP:731-748) for one pass: out = trk, then nlayer x
+ * (propagateToZ, KalmanUpdate_v2) per block, fused in one launch.  trk, hit,
+ * out are device pointers (MPTRK[nevts*nb], MPHIT[nlayer*nevts*nb],
+ * MPTRK[nevts*nb]).  Asynchronous. */
+int p2z_run(p2z_ctx* ctx, const void* trk, const void* hit, void* out, int64_t nevts, int64_t nb,
+            int32_t nlayer, int32_t width, int32_t math);
+/* Same with host pointers (the reference's include-data mode, CU4:830-875):
+ * chunks of events are copied in, computed and copied out on the context's
+ * streams so that transfers overlap compute.  Returns when out is complete.
+ * Pinned buffers (p2z_host_alloc) give full PCIe rate. */
+int p2z_run_host(p2z_ctx* ctx, const void* trk, const void* hit, void* out, int64_t nevts,
+                 int64_t nb, int32_t nlayer, int32_t width, int32_t math);
+
+/* ---- report (device) ------------------------------------------------------
+ * The statistics of OMP:758-876 computed on the GPU from device-resident
+ * output: double accumulation in a fixed order (deterministic), same
+ * acceptance window and denominators.  lasthit = unsmeared hit of the last
+ * layer.  ntrks <= nb*W.  Synchronous. */
+typedef struct {
+  double avgx, avgy, avgz, avgdx, avgdy, avgdz, avgpt, avgphi, avgtheta;
+  double stdx, stdy, stdz, stddx, stddy, stddz;
+  int32_t nnans, nfail, ngood, reserved;
+} p2z_summary;
+int p2z_summary_device(p2z_ctx* ctx, const void* out, int64_t nevts, int64_t nb, int32_t width,
+                       int64_t ntrks, const float lasthit[3], p2z_summary* s);
+/* The same statistics on host memory (used by the drop-in report). */
+int p2z_summary_host(const void* out, int64_t nevts, int64_t nb, int32_t width, int64_t ntrks,
+                     const float lasthit[3], p2z_summary* s);
+/* The 11 report lines of OMP:866-876 into buf; returns length or P2Z_EINVAL. */
+int p2z_format_summary(const p2z_summary* s, char* buf, int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* P2Z_H */

@@ -1,0 +1,355 @@
+// C ABI of the p2z library: contexts, memory, launches (declared in include/p2z.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "p2z.h"
+#include "p2z_capi_util.hpp"
+#include "p2z_internal.hpp"
+
+using p2z::LaunchCfg;
+
+struct p2z_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;            // main stream: device-pointer entry points
+  std::vector<hipStream_t> pipe;           // host-buffer pipeline streams
+  hipEvent_t t0 = nullptr, t1 = nullptr;   // timer
+  LaunchCfg cfg;
+  // host-buffer pipeline staging, one slot per pipeline stream
+  std::vector<void*> slot_trk, slot_hit, slot_out;
+  size_t slot_trk_bytes = 0, slot_hit_bytes = 0;
+  // device report scratch
+  double* sum_part = nullptr;
+  size_t sum_part_len = 0;
+  double* sum_res = nullptr;
+};
+
+namespace {
+
+int use_device(p2z_ctx* c) {
+  P2Z_HIP_CHECK(hipSetDevice(c->device));
+  return P2Z_OK;
+}
+
+int apply_opts(p2z_ctx* c, const p2z_opts* o) {
+  LaunchCfg cfg;
+  int ns = 4;
+  if (o) {
+    if (o->tracks_per_wg != 0) {
+      if (o->tracks_per_wg < 64 || o->tracks_per_wg > 256 || o->tracks_per_wg % 64 != 0)
+        return p2z::fail(P2Z_EINVAL, "tracks_per_wg must be 64, 128, 192 or 256 (got %d)",
+                         o->tracks_per_wg);
+      cfg.tracks_per_wg = o->tracks_per_wg;
+    }
+    if (o->nstreams != 0) {
+      if (o->nstreams < 1 || o->nstreams > 16)
+        return p2z::fail(P2Z_EINVAL, "nstreams must be in [1,16] (got %d)", o->nstreams);
+      ns = o->nstreams;
+    }
+    if (o->hit_stage < 0 || o->hit_stage > 8)
+      return p2z::fail(P2Z_EINVAL, "hit_stage must be in [0,8] (got %d)", o->hit_stage);
+    cfg.hit_stage = o->hit_stage;
+  }
+  c->cfg = cfg;
+  if ((int)c->pipe.size() != ns) {
+    for (auto s : c->pipe) (void)hipStreamDestroy(s);
+    for (auto p : c->slot_trk) (void)hipFree(p);
+    for (auto p : c->slot_hit) (void)hipFree(p);
+    for (auto p : c->slot_out) (void)hipFree(p);
+    c->slot_trk.clear(); c->slot_hit.clear(); c->slot_out.clear();
+    c->slot_trk_bytes = c->slot_hit_bytes = 0;
+    c->pipe.assign(ns, nullptr);
+    for (auto& s : c->pipe) P2Z_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  }
+  return P2Z_OK;
+}
+
+int check_blocks(const void* a, const void* b, const void* c, int64_t nevts, int64_t nb,
+                 int32_t nlayer, int32_t width, int32_t math) {
+  if (!a || !b || !c) return p2z::fail(P2Z_EINVAL, "null buffer");
+  if (nevts < 0 || nb < 0 || nlayer < 0)
+    return p2z::fail(P2Z_EINVAL, "negative size (nevts=%lld nb=%lld nlayer=%d)", (long long)nevts,
+                     (long long)nb, nlayer);
+  if (!p2z::valid_width(width)) return p2z::fail(P2Z_EINVAL, "width must be 16/32/64/128 (got %d)", width);
+  if (!p2z::valid_math(math)) return p2z::fail(P2Z_EINVAL, "unknown math mode %d", math);
+  if (nevts * nb * (int64_t)width > (int64_t)1 << 31)
+    return p2z::fail(P2Z_EINVAL, "more than 2^31 tracks in one call");
+  return P2Z_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int p2z_version(void) { return P2Z_ABI_VERSION; }
+
+int p2z_device_count(int* n) {
+  if (!n) return p2z::fail(P2Z_EINVAL, "null argument");
+  *n = 0;
+  const hipError_t e = hipGetDeviceCount(n);
+  if (e == hipErrorNoDevice) { *n = 0; return P2Z_OK; }
+  P2Z_HIP_CHECK(e);
+  return P2Z_OK;
+}
+
+int p2z_create(p2z_ctx** out, int device, const p2z_opts* opts) {
+  if (!out) return p2z::fail(P2Z_EINVAL, "null ctx pointer");
+  *out = nullptr;
+  int n = 0;
+  const hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || device < 0 || device >= n)
+    return p2z::fail(P2Z_ENODEV, "no GPU %d (%d visible)", device, e == hipSuccess ? n : 0);
+  p2z_ctx* c = new p2z_ctx;
+  c->device = device;
+  int rc = use_device(c);
+  if (rc == P2Z_OK) rc = p2z::hip_status(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  if (rc == P2Z_OK) rc = p2z::hip_status(hipEventCreate(&c->t0), "hipEventCreate");
+  if (rc == P2Z_OK) rc = p2z::hip_status(hipEventCreate(&c->t1), "hipEventCreate");
+  if (rc == P2Z_OK) rc = apply_opts(c, opts);
+  if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc((void**)&c->sum_res, 32 * sizeof(double)), "hipMalloc");
+  if (rc != P2Z_OK) {
+    p2z_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return P2Z_OK;
+}
+
+int p2z_destroy(p2z_ctx* c) {
+  if (!c) return P2Z_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto s : c->pipe) (void)hipStreamSynchronize(s);
+  for (auto p : c->slot_trk) (void)hipFree(p);
+  for (auto p : c->slot_hit) (void)hipFree(p);
+  for (auto p : c->slot_out) (void)hipFree(p);
+  for (auto s : c->pipe) (void)hipStreamDestroy(s);
+  if (c->sum_part) (void)hipFree(c->sum_part);
+  if (c->sum_res) (void)hipFree(c->sum_res);
+  if (c->t0) (void)hipEventDestroy(c->t0);
+  if (c->t1) (void)hipEventDestroy(c->t1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return P2Z_OK;
+}
+
+int p2z_set_opts(p2z_ctx* c, const p2z_opts* o) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  P2Z_TRY(use_device(c));
+  P2Z_HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (auto s : c->pipe) P2Z_HIP_CHECK(hipStreamSynchronize(s));
+  return apply_opts(c, o);
+}
+
+int p2z_sync(p2z_ctx* c) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  P2Z_TRY(use_device(c));
+  P2Z_HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (auto s : c->pipe) P2Z_HIP_CHECK(hipStreamSynchronize(s));
+  return P2Z_OK;
+}
+
+int p2z_get_stream(p2z_ctx* c, void** s) {
+  if (!c || !s) return p2z::fail(P2Z_EINVAL, "null argument");
+  *s = (void*)c->stream;
+  return P2Z_OK;
+}
+
+int p2z_dev_alloc(p2z_ctx* c, void** p, size_t bytes) {
+  if (!c || !p) return p2z::fail(P2Z_EINVAL, "null argument");
+  *p = nullptr;
+  P2Z_TRY(use_device(c));
+  if (bytes == 0) return P2Z_OK;
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e == hipErrorOutOfMemory) return p2z::fail(P2Z_ENOMEM, "hipMalloc(%zu) out of memory", bytes);
+  P2Z_HIP_CHECK(e);
+  return P2Z_OK;
+}
+
+int p2z_dev_free(p2z_ctx* c, void* p) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  if (!p) return P2Z_OK;
+  P2Z_TRY(use_device(c));
+  P2Z_HIP_CHECK(hipFree(p));
+  return P2Z_OK;
+}
+
+int p2z_host_alloc(p2z_ctx* c, void** p, size_t bytes) {
+  if (!c || !p) return p2z::fail(P2Z_EINVAL, "null argument");
+  *p = nullptr;
+  P2Z_TRY(use_device(c));
+  if (bytes == 0) return P2Z_OK;
+  const hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+  if (e == hipErrorOutOfMemory) return p2z::fail(P2Z_ENOMEM, "hipHostMalloc(%zu) out of memory", bytes);
+  P2Z_HIP_CHECK(e);
+  return P2Z_OK;
+}
+
+int p2z_host_free(p2z_ctx* c, void* p) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  if (!p) return P2Z_OK;
+  P2Z_HIP_CHECK(hipHostFree(p));
+  return P2Z_OK;
+}
+
+int p2z_memcpy_h2d(p2z_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return p2z::fail(P2Z_EINVAL, "null argument");
+  P2Z_TRY(use_device(c));
+  if (bytes) P2Z_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return P2Z_OK;
+}
+
+int p2z_memcpy_d2h(p2z_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return p2z::fail(P2Z_EINVAL, "null argument");
+  P2Z_TRY(use_device(c));
+  if (bytes) P2Z_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  return P2Z_OK;
+}
+
+int p2z_timer_start(p2z_ctx* c) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  P2Z_TRY(use_device(c));
+  P2Z_HIP_CHECK(hipEventRecord(c->t0, c->stream));
+  return P2Z_OK;
+}
+
+int p2z_timer_stop(p2z_ctx* c, float* ms) {
+  if (!c || !ms) return p2z::fail(P2Z_EINVAL, "null argument");
+  P2Z_TRY(use_device(c));
+  P2Z_HIP_CHECK(hipEventRecord(c->t1, c->stream));
+  P2Z_HIP_CHECK(hipEventSynchronize(c->t1));
+  P2Z_HIP_CHECK(hipEventElapsedTime(ms, c->t0, c->t1));
+  return P2Z_OK;
+}
+
+int p2z_propagate_to_z(p2z_ctx* c, const float* inErr, const float* inPar, const int32_t* inChg,
+                       const float* msP, float* outErr, float* outPar, int64_t nblocks,
+                       int64_t trk_stride, int64_t hit_stride, int32_t width, int32_t math) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  if (!inErr || !inPar || !inChg || !msP || !outErr || !outPar)
+    return p2z::fail(P2Z_EINVAL, "null buffer");
+  if (!p2z::valid_width(width)) return p2z::fail(P2Z_EINVAL, "width must be 16/32/64/128 (got %d)", width);
+  if (!p2z::valid_math(math)) return p2z::fail(P2Z_EINVAL, "unknown math mode %d", math);
+  if (nblocks < 0 || nblocks * (int64_t)width > (int64_t)1 << 31)
+    return p2z::fail(P2Z_EINVAL, "bad block count %lld", (long long)nblocks);
+  if (trk_stride < 0 || hit_stride < 3 * width)
+    return p2z::fail(P2Z_EINVAL, "bad strides (trk %lld, hit %lld)", (long long)trk_stride, (long long)hit_stride);
+  P2Z_TRY(use_device(c));
+  P2Z_HIP_CHECK(p2z::launch_propagate(inErr, inPar, inChg, msP, outErr, outPar, nblocks, trk_stride,
+                                      hit_stride, width, math, c->cfg, c->stream));
+  return P2Z_OK;
+}
+
+int p2z_kalman_update(p2z_ctx* c, float* trkErr, float* inPar, const float* hitErr, const float* msP,
+                      int64_t nblocks, int64_t trk_stride, int64_t hit_stride, int32_t width,
+                      int32_t math) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  if (!trkErr || !inPar || !hitErr || !msP) return p2z::fail(P2Z_EINVAL, "null buffer");
+  if (!p2z::valid_width(width)) return p2z::fail(P2Z_EINVAL, "width must be 16/32/64/128 (got %d)", width);
+  if (!p2z::valid_math(math)) return p2z::fail(P2Z_EINVAL, "unknown math mode %d", math);
+  if (nblocks < 0 || nblocks * (int64_t)width > (int64_t)1 << 31)
+    return p2z::fail(P2Z_EINVAL, "bad block count %lld", (long long)nblocks);
+  if (trk_stride < 0 || hit_stride < 0)
+    return p2z::fail(P2Z_EINVAL, "bad strides (trk %lld, hit %lld)", (long long)trk_stride, (long long)hit_stride);
+  P2Z_TRY(use_device(c));
+  P2Z_HIP_CHECK(p2z::launch_update(trkErr, inPar, hitErr, msP, nblocks, trk_stride, hit_stride, width,
+                                   math, c->cfg, c->stream));
+  return P2Z_OK;
+}
+
+int p2z_run(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_t nevts, int64_t nb,
+            int32_t nlayer, int32_t width, int32_t math) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  P2Z_TRY(check_blocks(trk, hit, out, nevts, nb, nlayer, width, math));
+  P2Z_TRY(use_device(c));
+  P2Z_HIP_CHECK(p2z::launch_fused((const float*)trk, (const float*)hit, (float*)out, nevts * nb, nlayer,
+                                  width, math, c->cfg, c->stream));
+  return P2Z_OK;
+}
+
+int p2z_run_host(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_t nevts, int64_t nb,
+                 int32_t nlayer, int32_t width, int32_t math) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  P2Z_TRY(check_blocks(trk, hit, out, nevts, nb, nlayer, width, math));
+  P2Z_TRY(use_device(c));
+  if (nevts == 0 || nb == 0) return P2Z_OK;
+  const int ns = (int)c->pipe.size();
+  // Chunks of whole events, two per stream so that copies of one chunk overlap
+  // the compute of the previous one on the other streams.
+  const int64_t nchunks = std::min<int64_t>(nevts, 2 * (int64_t)ns);
+  const int64_t per = (nevts + nchunks - 1) / nchunks;
+  const size_t trk_ev = (size_t)nb * p2z::kTrkWords * width * sizeof(float);
+  const size_t hit_ev = (size_t)nb * nlayer * p2z::kHitWords * width * sizeof(float);
+  const size_t need_trk = trk_ev * per, need_hit = std::max<size_t>(hit_ev * per, 1);
+  if (c->slot_trk_bytes < need_trk || c->slot_hit_bytes < need_hit || (int)c->slot_trk.size() != ns) {
+    for (auto s : c->pipe) P2Z_HIP_CHECK(hipStreamSynchronize(s));
+    for (auto p : c->slot_trk) (void)hipFree(p);
+    for (auto p : c->slot_hit) (void)hipFree(p);
+    for (auto p : c->slot_out) (void)hipFree(p);
+    c->slot_trk.assign(ns, nullptr); c->slot_hit.assign(ns, nullptr); c->slot_out.assign(ns, nullptr);
+    c->slot_trk_bytes = c->slot_hit_bytes = 0;
+    for (int i = 0; i < ns; ++i) {
+      hipError_t e = hipMalloc(&c->slot_trk[i], need_trk);
+      if (e == hipSuccess) e = hipMalloc(&c->slot_hit[i], need_hit);
+      if (e == hipSuccess) e = hipMalloc(&c->slot_out[i], need_trk);
+      if (e == hipErrorOutOfMemory) return p2z::fail(P2Z_ENOMEM, "staging buffers do not fit");
+      P2Z_HIP_CHECK(e);
+    }
+    c->slot_trk_bytes = need_trk;
+    c->slot_hit_bytes = need_hit;
+  }
+  for (int64_t k = 0; k < nchunks; ++k) {
+    const int64_t e0 = k * per, e1 = std::min(nevts, e0 + per);
+    if (e0 >= e1) break;
+    const int i = (int)(k % ns);
+    hipStream_t s = c->pipe[i];
+    const int64_t ne = e1 - e0;
+    P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_trk[i], (const char*)trk + e0 * trk_ev, ne * trk_ev,
+                                 hipMemcpyHostToDevice, s));
+    P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_hit[i], (const char*)hit + e0 * hit_ev, ne * hit_ev,
+                                 hipMemcpyHostToDevice, s));
+    P2Z_HIP_CHECK(p2z::launch_fused((const float*)c->slot_trk[i], (const float*)c->slot_hit[i],
+                                    (float*)c->slot_out[i], ne * nb, nlayer, width, math, c->cfg, s));
+    P2Z_HIP_CHECK(hipMemcpyAsync((char*)out + e0 * trk_ev, c->slot_out[i], ne * trk_ev,
+                                 hipMemcpyDeviceToHost, s));
+  }
+  for (auto s : c->pipe) P2Z_HIP_CHECK(hipStreamSynchronize(s));
+  return P2Z_OK;
+}
+
+int p2z_summary_device(p2z_ctx* c, const void* out, int64_t nevts, int64_t nb, int32_t width,
+                       int64_t ntrks, const float lasthit[3], p2z_summary* s) {
+  if (!c || !out || !lasthit || !s) return p2z::fail(P2Z_EINVAL, "null argument");
+  if (!p2z::valid_width(width)) return p2z::fail(P2Z_EINVAL, "width must be 16/32/64/128 (got %d)", width);
+  if (nevts <= 0 || ntrks <= 0 || ntrks > nb * width)
+    return p2z::fail(P2Z_EINVAL, "need nevts > 0 and 0 < ntrks <= nb*width");
+  P2Z_TRY(use_device(c));
+  const size_t need = p2z::summary_scratch_doubles(nevts, ntrks);
+  if (c->sum_part_len < need) {
+    if (c->sum_part) P2Z_HIP_CHECK(hipFree(c->sum_part));
+    c->sum_part = nullptr;
+    c->sum_part_len = 0;
+    P2Z_HIP_CHECK(hipMalloc((void**)&c->sum_part, need * sizeof(double)));
+    c->sum_part_len = need;
+  }
+  P2Z_HIP_CHECK(p2z::launch_summary((const float*)out, nevts, nb, width, ntrks, lasthit[0], lasthit[1],
+                                    lasthit[2], c->sum_part, c->sum_res, c->stream));
+  double r[18];
+  P2Z_HIP_CHECK(hipMemcpyAsync(r, c->sum_res, sizeof r, hipMemcpyDeviceToHost, c->stream));
+  P2Z_HIP_CHECK(hipStreamSynchronize(c->stream));
+  std::memset(s, 0, sizeof *s);
+  s->avgx = r[0]; s->avgy = r[1]; s->avgz = r[2];
+  s->avgdx = r[3]; s->avgdy = r[4]; s->avgdz = r[5];
+  s->avgpt = r[6]; s->avgphi = r[7]; s->avgtheta = r[8];
+  s->nnans = (int32_t)r[9]; s->nfail = (int32_t)r[10]; s->ngood = (int32_t)r[11];
+  s->stdx = r[12]; s->stdy = r[13]; s->stdz = r[14];
+  s->stddx = r[15]; s->stddy = r[16]; s->stddz = r[17];
+  return P2Z_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,42 @@
+// Internal interface between the C-ABI layer (p2z_capi.hip) and the kernel
+// translation unit (p2z_kernels.hip).  Not installed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace p2z {
+
+// Words per AoSoA block of width W (reference MPTRK / MPHIT, OMP:115-125).
+constexpr int kTrkWords = 28;
+constexpr int kHitWords = 9;
+
+inline bool valid_width(int w) { return w == 16 || w == 32 || w == 64 || w == 128; }
+inline bool valid_math(int m) { return m >= 0 && m <= 2; }
+
+struct LaunchCfg {
+  int tracks_per_wg = 256;  // threads per workgroup, one track per thread
+  int hit_stage = 0;        // LDS staging depth in layers (0: register prefetch)
+};
+
+// Fused driver pass: ntracks = nblocks*W tracks, all with nlayer layers.
+hipError_t launch_fused(const float* trk, const float* hit, float* out, int64_t nblocks,
+                        int nlayer, int width, int math, const LaunchCfg& cfg, hipStream_t s);
+
+hipError_t launch_propagate(const float* inErr, const float* inPar, const int32_t* inChg,
+                            const float* msP, float* outErr, float* outPar, int64_t nblocks,
+                            int64_t ts, int64_t hs, int width, int math, const LaunchCfg& cfg,
+                            hipStream_t s);
+
+hipError_t launch_update(float* trkErr, float* inPar, const float* hitErr, const float* msP,
+                         int64_t nblocks, int64_t ts, int64_t hs, int width, int math,
+                         const LaunchCfg& cfg, hipStream_t s);
+
+// Report statistics (OMP:758-851) on device.  part: scratch of
+// summary_scratch_doubles(nevts, ntrks) doubles.  res: 16 doubles on device
+// (9 sums / 6 sums of squares, counts as doubles), written by the final pass.
+size_t summary_scratch_doubles(int64_t nevts, int64_t ntrks);
+hipError_t launch_summary(const float* out, int64_t nevts, int64_t nb, int width, int64_t ntrks,
+                          double hx, double hy, double hz, double* part, double* res,
+                          hipStream_t s);
+
+}  // namespace p2z

@@ -1,0 +1,361 @@
+// gfx950 kernels of the propagate-to-z + Kalman-update path.
+//
+// Mapping: one track per lane (wave64), track g = blockIdx.x*blockDim.x +
+// threadIdx.x lives in AoSoA block b = g / W at lane l = g % W.  Every load and
+// store of a wave touches 64/W contiguous runs of 4W bytes (element-major
+// AoSoA, reference OMP:159-161), so track and hit traffic is fully coalesced
+// without any transpose.  The track state stays in VGPRs for all layers; the
+// hits of layer L+1 are loaded before layer L is computed.
+#include "p2z_internal.hpp"
+#include "p2z_lane.hpp"
+
+namespace p2z {
+namespace {
+
+template <int W>
+__device__ inline void load_track(Track& t, const float* __restrict__ tb) {
+#pragma unroll
+  for (int e = 0; e < 6; ++e) t.p[e] = tb[e * W];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) t.P[e] = tb[(6 + e) * W];
+  t.q = reinterpret_cast<const int32_t*>(tb)[27 * W];
+}
+
+template <int W>
+__device__ inline void store_track(const Track& t, float* __restrict__ ob) {
+#pragma unroll
+  for (int e = 0; e < 6; ++e) ob[e * W] = t.p[e];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) ob[(6 + e) * W] = t.P[e];
+  reinterpret_cast<int32_t*>(ob)[27 * W] = t.q;
+}
+
+// Hits are read exactly once: non-temporal loads keep them out of the way of
+// the track state in the caches.
+template <int W>
+__device__ inline Hit load_hit(const float* __restrict__ hb) {
+  Hit h;
+  h.x = __builtin_nontemporal_load(hb + 0 * W);
+  h.y = __builtin_nontemporal_load(hb + 1 * W);
+  h.z = __builtin_nontemporal_load(hb + 2 * W);
+  h.xx = __builtin_nontemporal_load(hb + 3 * W);
+  h.xy = __builtin_nontemporal_load(hb + 4 * W);
+  h.yy = __builtin_nontemporal_load(hb + 5 * W);
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// fused driver pass (reference OMP:733-747 for every block at once)
+// ---------------------------------------------------------------------------
+template <int W, int kMath>
+__global__ __launch_bounds__(256) void k_fused(const float* __restrict__ trk,
+                                               const float* __restrict__ hit,
+                                               float* __restrict__ out, int64_t ntracks,
+                                               int nlayer) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntracks) return;
+  const int64_t b = g / W;
+  const int l = (int)(g % W);
+  Track t;
+  load_track<W>(t, trk + b * (kTrkWords * W) + l);
+  const float* hb = hit + b * (int64_t)nlayer * (kHitWords * W) + l;
+  if (nlayer > 0) {
+    Hit h = load_hit<W>(hb);
+    Hit hn = h;
+    if (nlayer > 1) hn = load_hit<W>(hb + kHitWords * W);
+    layer_step<kMath, true>(t, h);
+    for (int lay = 1; lay < nlayer; ++lay) {
+      h = hn;
+      if (lay + 1 < nlayer) hn = load_hit<W>(hb + (int64_t)(lay + 1) * (kHitWords * W));
+      layer_step<kMath, false>(t, h);
+    }
+  }
+  store_track<W>(t, out + b * (kTrkWords * W) + l);
+}
+
+// ---------------------------------------------------------------------------
+// per-function kernels (reference propagateToZ / KalmanUpdate_v2 on blocks)
+// ---------------------------------------------------------------------------
+template <int W, int kMath>
+__global__ __launch_bounds__(256) void k_propagate(const float* inErr, const float* inPar,
+                                                   const int32_t* inChg, const float* msP,
+                                                   float* outErr, float* outPar,
+                                                   int64_t ntracks, int64_t ts, int64_t hs) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntracks) return;
+  const int64_t b = g / W;
+  const int l = (int)(g % W);
+  Track t;
+  const int64_t o = b * ts + l;
+#pragma unroll
+  for (int e = 0; e < 21; ++e) t.P[e] = inErr[o + e * W];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) t.p[e] = inPar[o + e * W];
+  t.q = inChg[o];
+  const float hz = msP[b * hs + 2 * W + l];
+  if constexpr (kMath == kRef)
+    propagate_ref(t, hz);
+  else
+    propagate_fast<false>(t, hz);
+#pragma unroll
+  for (int e = 0; e < 21; ++e) outErr[o + e * W] = t.P[e];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) outPar[o + e * W] = t.p[e];
+}
+
+template <int W, int kMath>
+__global__ __launch_bounds__(256) void k_update(float* trkErr, float* inPar, const float* hitErr,
+                                                const float* msP, int64_t ntracks, int64_t ts,
+                                                int64_t hs) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntracks) return;
+  const int64_t b = g / W;
+  const int l = (int)(g % W);
+  Track t;
+  const int64_t o = b * ts + l;
+#pragma unroll
+  for (int e = 0; e < 21; ++e) t.P[e] = trkErr[o + e * W];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) t.p[e] = inPar[o + e * W];
+  t.q = 0;
+  const int64_t ho = b * hs + l;
+  Hit h;
+  h.x = msP[ho];
+  h.y = msP[ho + W];
+  h.z = msP[ho + 2 * W];
+  h.xx = hitErr[ho];
+  h.xy = hitErr[ho + W];
+  h.yy = hitErr[ho + 2 * W];
+  if constexpr (kMath == kRef)
+    update_ref(t, h);
+  else if constexpr (kMath == kFast)
+    update_fast<false>(t, h);
+  else
+    update_fp64gain<false>(t, h);
+#pragma unroll
+  for (int e = 0; e < 21; ++e) trkErr[o + e * W] = t.P[e];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) inPar[o + e * W] = t.p[e];
+}
+
+// ---------------------------------------------------------------------------
+// report statistics (OMP:758-851), deterministic fixed-order reduction
+// ---------------------------------------------------------------------------
+constexpr int kSumThreads = 256;
+constexpr int kSumVals = 12;  // pass 1: 9 sums + nnans, nfail, ngood; pass 2: 6 sums
+
+struct TrackView {
+  double x, y, z, pt, phi, theta;
+  int state;  // 0 good, 1 non-finite, 2 outside the acceptance window
+};
+
+__device__ inline TrackView view_track(const float* out, int64_t nb, int W, int64_t ntrks,
+                                       int64_t idx, double hx, double hy, double hz) {
+  const int64_t ie = idx / ntrks, it = idx % ntrks;
+  const float* blk = out + ((it / W) + nb * ie) * (int64_t)(kTrkWords * W);
+  const int64_t l = it % W;
+  TrackView v;
+  v.x = blk[0 * W + l];
+  v.y = blk[1 * W + l];
+  v.z = blk[2 * W + l];
+  v.pt = 1. / (double)blk[3 * W + l];
+  v.phi = blk[4 * W + l];
+  v.theta = blk[5 * W + l];
+  if (!isfinite(v.x) || !isfinite(v.y) || !isfinite(v.z) || !isfinite(v.pt) ||
+      !isfinite(v.phi) || !isfinite(v.theta)) {
+    v.state = 1;
+  } else if (fabs((v.x - hx) / hx) > 1. || fabs((v.y - hy) / hy) > 1. ||
+             fabs((v.z - hz) / hz) > 1. || fabs((v.pt - 12.) / 12.) > 1. ||
+             fabs((v.phi - 1.3) / 1.3) > 1. || fabs((v.theta - 2.8) / 2.8) > 1.) {
+    v.state = 2;
+  } else {
+    v.state = 0;
+  }
+  return v;
+}
+
+// Tree-reduce kSumVals doubles per thread over the workgroup in a fixed order;
+// thread 0 returns the totals in vals.
+__device__ inline void block_reduce(double (&vals)[kSumVals], double* lds) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kSumVals; ++k) lds[k * kSumThreads + t] = vals[k];
+  __syncthreads();
+  for (int stride = kSumThreads / 2; stride > 0; stride >>= 1) {
+    if (t < stride) {
+#pragma unroll
+      for (int k = 0; k < kSumVals; ++k) lds[k * kSumThreads + t] += lds[k * kSumThreads + t + stride];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < kSumVals; ++k) vals[k] = lds[k * kSumThreads];
+}
+
+template <int W, int kPass>
+__global__ __launch_bounds__(kSumThreads) void k_summary(const float* __restrict__ out, int64_t nb,
+                                                         int64_t ntrks, int64_t total, double hx,
+                                                         double hy, double hz,
+                                                         const double* __restrict__ res,
+                                                         double* __restrict__ part) {
+  __shared__ double lds[kSumVals * kSumThreads];
+  double v[kSumVals];
+#pragma unroll
+  for (int k = 0; k < kSumVals; ++k) v[k] = 0.;
+  const int64_t idx = (int64_t)blockIdx.x * kSumThreads + threadIdx.x;
+  if (idx < total) {
+    const TrackView tv = view_track(out, nb, W, ntrks, idx, hx, hy, hz);
+    if (kPass == 1) {
+      if (tv.state == 0) {
+        v[0] = tv.x; v[1] = tv.y; v[2] = tv.z;
+        v[3] = (tv.x - hx) / tv.x; v[4] = (tv.y - hy) / tv.y; v[5] = (tv.z - hz) / tv.z;
+        v[6] = tv.pt; v[7] = tv.phi; v[8] = tv.theta;
+        v[11] = 1.;
+      } else if (tv.state == 1) {
+        v[9] = 1.;
+      } else {
+        v[10] = 1.;
+      }
+    } else if (tv.state == 0) {
+      const double dx = tv.x - res[0], dy = tv.y - res[1], dz = tv.z - res[2];
+      const double ddx = (tv.x - hx) / tv.x - res[3], ddy = (tv.y - hy) / tv.y - res[4],
+                   ddz = (tv.z - hz) / tv.z - res[5];
+      v[0] = dx * dx; v[1] = dy * dy; v[2] = dz * dz;
+      v[3] = ddx * ddx; v[4] = ddy * ddy; v[5] = ddz * ddz;
+    }
+  }
+  block_reduce(v, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < kSumVals; ++k) part[(int64_t)blockIdx.x * kSumVals + k] = v[k];
+  }
+}
+
+// Final reduction of the per-workgroup partials (one workgroup).  Pass 1 turns
+// the sums into means (res[0..8]) and counts (res[9..11] = nnans, nfail, ngood);
+// pass 2 into the rounded-to-float RMS values (res[12..17]).
+template <int kPass>
+__global__ __launch_bounds__(kSumThreads) void k_summary_final(const double* __restrict__ part,
+                                                               int64_t nparts,
+                                                               double* __restrict__ res) {
+  __shared__ double lds[kSumVals * kSumThreads];
+  double v[kSumVals];
+#pragma unroll
+  for (int k = 0; k < kSumVals; ++k) v[k] = 0.;
+  for (int64_t i = threadIdx.x; i < nparts; i += kSumThreads) {
+#pragma unroll
+    for (int k = 0; k < kSumVals; ++k) v[k] += part[i * kSumVals + k];
+  }
+  block_reduce(v, lds);
+  if (threadIdx.x == 0) {
+    if (kPass == 1) {
+      const double n = v[11];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) res[k] = v[k] / n;
+      res[9] = v[9];
+      res[10] = v[10];
+      res[11] = v[11];
+    } else {
+      const double n = res[11];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) res[12 + k] = (double)sqrtf((float)(v[k] / n));
+    }
+  }
+}
+
+template <typename F>
+hipError_t dispatch_width(int width, F&& f) {
+  switch (width) {
+    case 16: return f(std::integral_constant<int, 16>{});
+    case 32: return f(std::integral_constant<int, 32>{});
+    case 64: return f(std::integral_constant<int, 64>{});
+    case 128: return f(std::integral_constant<int, 128>{});
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename F>
+hipError_t dispatch_math(int math, F&& f) {
+  switch (math) {
+    case kFast: return f(std::integral_constant<int, kFast>{});
+    case kRef: return f(std::integral_constant<int, kRef>{});
+    case kFp64Gain: return f(std::integral_constant<int, kFp64Gain>{});
+  }
+  return hipErrorInvalidValue;
+}
+
+inline int64_t grid_for(int64_t n, int tpw) { return (n + tpw - 1) / tpw; }
+
+}  // namespace
+
+hipError_t launch_fused(const float* trk, const float* hit, float* out, int64_t nblocks, int nlayer,
+                        int width, int math, const LaunchCfg& cfg, hipStream_t s) {
+  const int64_t ntracks = nblocks * width;
+  if (ntracks == 0) return hipSuccess;
+  const int tpw = cfg.tracks_per_wg;
+  return dispatch_width(width, [&](auto wc) {
+    return dispatch_math(math, [&](auto mc) {
+      constexpr int W = decltype(wc)::value, M = decltype(mc)::value;
+      hipLaunchKernelGGL((k_fused<W, M>), dim3((unsigned)grid_for(ntracks, tpw)), dim3(tpw), 0, s,
+                         trk, hit, out, ntracks, nlayer);
+      return hipGetLastError();
+    });
+  });
+}
+
+hipError_t launch_propagate(const float* inErr, const float* inPar, const int32_t* inChg,
+                            const float* msP, float* outErr, float* outPar, int64_t nblocks,
+                            int64_t ts, int64_t hs, int width, int math, const LaunchCfg& cfg,
+                            hipStream_t s) {
+  const int64_t ntracks = nblocks * width;
+  if (ntracks == 0) return hipSuccess;
+  const int tpw = cfg.tracks_per_wg;
+  return dispatch_width(width, [&](auto wc) {
+    return dispatch_math(math, [&](auto mc) {
+      constexpr int W = decltype(wc)::value, M = decltype(mc)::value;
+      hipLaunchKernelGGL((k_propagate<W, M>), dim3((unsigned)grid_for(ntracks, tpw)), dim3(tpw), 0,
+                         s, inErr, inPar, inChg, msP, outErr, outPar, ntracks, ts, hs);
+      return hipGetLastError();
+    });
+  });
+}
+
+hipError_t launch_update(float* trkErr, float* inPar, const float* hitErr, const float* msP,
+                         int64_t nblocks, int64_t ts, int64_t hs, int width, int math,
+                         const LaunchCfg& cfg, hipStream_t s) {
+  const int64_t ntracks = nblocks * width;
+  if (ntracks == 0) return hipSuccess;
+  const int tpw = cfg.tracks_per_wg;
+  return dispatch_width(width, [&](auto wc) {
+    return dispatch_math(math, [&](auto mc) {
+      constexpr int W = decltype(wc)::value, M = decltype(mc)::value;
+      hipLaunchKernelGGL((k_update<W, M>), dim3((unsigned)grid_for(ntracks, tpw)), dim3(tpw), 0, s,
+                         trkErr, inPar, hitErr, msP, ntracks, ts, hs);
+      return hipGetLastError();
+    });
+  });
+}
+
+size_t summary_scratch_doubles(int64_t nevts, int64_t ntrks) {
+  return (size_t)grid_for(nevts * ntrks, kSumThreads) * kSumVals;
+}
+
+hipError_t launch_summary(const float* out, int64_t nevts, int64_t nb, int width, int64_t ntrks,
+                          double hx, double hy, double hz, double* part, double* res,
+                          hipStream_t s) {
+  const int64_t total = nevts * ntrks;
+  const int64_t nparts = grid_for(total, kSumThreads);
+  if (nparts == 0) return hipErrorInvalidValue;
+  return dispatch_width(width, [&](auto wc) {
+    constexpr int W = decltype(wc)::value;
+    hipLaunchKernelGGL((k_summary<W, 1>), dim3((unsigned)nparts), dim3(kSumThreads), 0, s, out, nb,
+                       ntrks, total, hx, hy, hz, res, part);
+    hipLaunchKernelGGL((k_summary_final<1>), dim3(1), dim3(kSumThreads), 0, s, part, nparts, res);
+    hipLaunchKernelGGL((k_summary<W, 2>), dim3((unsigned)nparts), dim3(kSumThreads), 0, s, out, nb,
+                       ntrks, total, hx, hy, hz, res, part);
+    hipLaunchKernelGGL((k_summary_final<2>), dim3(1), dim3(kSumThreads), 0, s, part, nparts, res);
+    return hipGetLastError();
+  });
+}
+
+}  // namespace p2z

@@ -1,0 +1,434 @@
+// Per-track (per-lane) device math of the propagate-to-z + Kalman-update path.
+//
+// One GPU lane owns one track: its 6 parameters (x, y, z, ipt, phi, theta), its
+// packed symmetric 6x6 covariance P[21] (lower triangle, row-major) and its
+// charge stay in VGPRs for the whole layer loop.  Three arithmetic modes:
+//
+//   kRef      : the reference CPU typing (src/propagate-toz-test_OMP.cpp:
+//               propagateToZ 623-682, MultHelixPropEndcap 296-341,
+//               MultHelixPropTranspEndcap 343-373, KalmanUpdate_v2 503-620):
+//               FP32 with the FP64 sub-expressions the reference's `1.`
+//               literals and `(double)` casts create, no contraction, the
+//               reference's summation order.
+//   kFast     : FP32 with FMA contraction, native reciprocals and the sparse
+//               structure of the Jacobian and of the covariance exploited.
+//   kFp64Gain : kFast propagation; the 2x2 residual-covariance inverse, the
+//               gain, the parameter update and a Joseph-form covariance update
+//               (I-KH) P (I-KH)^T + K R K^T in FP64.
+//
+// Structure used by kFast / kFp64Gain (exact for finite inputs):
+//   * the propagation Jacobian Jt is the identity plus 11 entries with row 2
+//     (z) zero, so after any propagation the z row/column of P is exactly 0;
+//   * the update therefore sees P(2,.) = 0, its gain row 2 is 0 and it leaves
+//     z and P(2,.) unchanged.
+// Only the first propagation of a track can see a non-zero z row (its input
+// covariance); later layers use the 15-entry form.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace p2z {
+
+enum Math : int { kFast = 0, kRef = 1, kFp64Gain = 2 };
+
+// kfact as the reference declares it: a float initialised from a double
+// expression (OMP:622).
+constexpr float kKfact = 100. / (-0.299792458 * 3.8112);
+
+// Packed lower-triangular index of (i, j) (reference SymOffsets66, OMP:52-55).
+__host__ __device__ constexpr int S(int i, int j) {
+  return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+}
+
+struct Track {
+  float p[6];   // x y z ipt phi theta
+  float P[21];  // packed lower triangle
+  int q;
+};
+
+struct Hit {
+  float x, y, z;     // position
+  float xx, xy, yy;  // covariance elements 0..2 (xz, yz, zz are never read)
+};
+
+// ---------------------------------------------------------------------------
+// kRef: reference typing
+// ---------------------------------------------------------------------------
+
+__device__ inline void propagate_ref(Track& t, float hz) {
+#pragma clang fp contract(off)
+  float* p = t.p;
+  float* P = t.P;
+  const float zout = hz;
+  const float k = t.q * kKfact;
+  const float deltaZ = zout - p[2];
+  const float ipt = p[3];
+  const float pt = (float)(1. / (double)ipt);
+  float sinP, cosP, sinT, cosT;
+  sincosf(p[4], &sinP, &cosP);
+  sincosf(p[5], &sinT, &cosT);
+  const float pxin = cosP * pt;
+  const float pyin = sinP * pt;
+  const float icosT = (float)(1.0 / (double)cosT);
+  const float icosTk = icosT / k;
+  const float alpha = deltaZ * sinT * ipt * icosTk;
+  float sina, cosa;
+  sincosf(alpha, &sina, &cosa);
+  p[0] = (float)((double)p[0] + (double)k * ((double)(pxin * sina) - (double)pyin * (1. - (double)cosa)));
+  p[1] = (float)((double)p[1] + (double)k * ((double)(pyin * sina) + (double)pxin * (1. - (double)cosa)));
+  p[2] = zout;
+  p[4] = p[4] + alpha;
+
+  float sC, cC;
+  sincosf(cosP * sina, &sC, &cC);
+  const double omcC = 1. - (double)cC;
+  const float J02 = cosP * sinT * (sinP * cosa * sC - cosa) * icosT;
+  const float J03 = (float)((double)(cosP * sinT * deltaZ * cosa) * (1. - (double)(sinP * sC)) * (double)(icosT * pt)
+                            - (double)k * ((double)(cosP * sina) - (double)sinP * omcC) * (double)(pt * pt));
+  const float J04 = (float)((double)(k * pt) * ((double)(-sinP * sina + sinP * sinP * sina * sC) - (double)cosP * omcC));
+  const float J05 = (float)((double)(cosP * deltaZ * cosa) * (1. - (double)(sinP * sC)) * (double)(icosT * icosT));
+  const float J12 = cosa * sinT * (cosP * cosP * sC - sinP) * icosT;
+  const float J13 = (float)((double)(sinT * deltaZ * cosa * (cosP * cosP * sC + sinP) * (icosT * pt))
+                            - (double)k * ((double)(sinP * sina) + (double)cosP * omcC) * (double)(pt * pt));
+  const float J14 = (float)((double)(k * pt) * (-(double)sinP * omcC - (double)(sinP * cosP * sina * sC)
+                                                + (double)(cosP * sina)));
+  const float J15 = deltaZ * cosa * (cosP * cosP * sC + sinP) * (icosT * icosT);
+  const float J42 = -ipt * sinT * icosTk;
+  const float J43 = sinT * deltaZ * icosTk;
+  const float J45 = ipt * deltaZ * (icosT * icosTk);
+
+  // T = Jt P: rows 0, 1, 4 summed in the reference's order; row 2 is zero,
+  // rows 3 and 5 copy P.
+  float T0[6], T1[6], T4[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const float p0 = P[S(0, c)], p1 = P[S(1, c)], p2 = P[S(2, c)];
+    const float p3 = P[S(3, c)], p4 = P[S(4, c)], p5 = P[S(5, c)];
+    T0[c] = p0 + J02 * p2 + J03 * p3 + J04 * p4 + J05 * p5;
+    T1[c] = p1 + J12 * p2 + J13 * p3 + J14 * p4 + J15 * p5;
+    T4[c] = J42 * p2 + J43 * p3 + p4 + J45 * p5;
+  }
+  float T3[6], T5[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    T3[c] = P[S(3, c)];
+    T5[c] = P[S(5, c)];
+  }
+  // P' = T Jt^T.  Entries (2,0) and (2,1) are sums over the zero row of T:
+  // they are 0, or NaN when a Jacobian entry is not finite, as in the reference.
+  const float z0 = 0.f;
+  P[S(0, 0)] = T0[0] + T0[2] * J02 + T0[3] * J03 + T0[4] * J04 + T0[5] * J05;
+  P[S(1, 0)] = T1[0] + T1[2] * J02 + T1[3] * J03 + T1[4] * J04 + T1[5] * J05;
+  P[S(1, 1)] = T1[1] + T1[2] * J12 + T1[3] * J13 + T1[4] * J14 + T1[5] * J15;
+  P[S(2, 0)] = z0 + z0 * J02 + z0 * J03 + z0 * J04 + z0 * J05;
+  P[S(2, 1)] = z0 + z0 * J12 + z0 * J13 + z0 * J14 + z0 * J15;
+  P[S(2, 2)] = 0.f;
+  P[S(3, 0)] = T3[0] + T3[2] * J02 + T3[3] * J03 + T3[4] * J04 + T3[5] * J05;
+  P[S(3, 1)] = T3[1] + T3[2] * J12 + T3[3] * J13 + T3[4] * J14 + T3[5] * J15;
+  P[S(3, 2)] = 0.f;
+  P[S(3, 3)] = T3[3];
+  P[S(4, 0)] = T4[0] + T4[2] * J02 + T4[3] * J03 + T4[4] * J04 + T4[5] * J05;
+  P[S(4, 1)] = T4[1] + T4[2] * J12 + T4[3] * J13 + T4[4] * J14 + T4[5] * J15;
+  P[S(4, 2)] = 0.f;
+  P[S(4, 3)] = T4[3];
+  P[S(4, 4)] = T4[2] * J42 + T4[3] * J43 + T4[4] + T4[5] * J45;
+  P[S(5, 0)] = T5[0] + T5[2] * J02 + T5[3] * J03 + T5[4] * J04 + T5[5] * J05;
+  P[S(5, 1)] = T5[1] + T5[2] * J12 + T5[3] * J13 + T5[4] * J14 + T5[5] * J15;
+  P[S(5, 2)] = 0.f;
+  P[S(5, 3)] = T5[3];
+  P[S(5, 4)] = T5[2] * J42 + T5[3] * J43 + T5[4] + T5[5] * J45;
+  P[S(5, 5)] = T5[5];
+}
+
+__device__ inline void update_ref(Track& t, const Hit& h) {
+#pragma clang fp contract(off)
+  float* p = t.p;
+  float* P = t.P;
+  float r00 = P[0] + h.xx;
+  float r01 = P[1] + h.xy;
+  float r11 = P[2] + h.yy;
+  const double det = (double)r00 * (double)r11 - (double)r01 * (double)r01;
+  const float s = (float)(1. / det);
+  const float tmp = s * r11;
+  r01 = r01 * -s;
+  r11 = s * r00;
+  r00 = tmp;
+  float K0[6], K1[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const float a = P[S(i, 0)], b = P[S(i, 1)];
+    K0[i] = a * r00 + b * r01;
+    K1[i] = a * r01 + b * r11;
+  }
+  const float res0 = h.x - p[0];
+  const float res1 = h.y - p[1];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) p[i] = p[i] + K0[i] * res0 + K1[i] * res1;
+  float C[21];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) C[S(i, j)] = K0[i] * P[S(0, j)] + K1[i] * P[S(1, j)];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) P[e] = P[e] - C[e];
+}
+
+// ---------------------------------------------------------------------------
+// kFast: FP32 + FMA, sparse forms
+// ---------------------------------------------------------------------------
+
+__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// The 11 Jacobian entries and the new x, y, z, phi of one track.
+struct Jac {
+  float J02, J03, J04, J05, J12, J13, J14, J15, J42, J43, J45;
+};
+
+__device__ inline Jac helix_fast(Track& t, float hz) {
+#pragma clang fp contract(fast)
+  float* p = t.p;
+  const float k = (float)t.q * kKfact;
+  const float deltaZ = hz - p[2];
+  const float ipt = p[3];
+  const float pt = fast_rcp(ipt);
+  float sinP, cosP, sinT, cosT;
+  sincosf(p[4], &sinP, &cosP);
+  sincosf(p[5], &sinT, &cosT);
+  const float icosT = fast_rcp(cosT);
+  const float icosTk = icosT * fast_rcp(k);
+  const float alpha = deltaZ * sinT * ipt * icosTk;
+  float sina, cosa;
+  sincosf(alpha, &sina, &cosa);
+  const float omca = 1.f - cosa;  // exact (Sterbenz) for |alpha| < pi/3
+  const float pxin = cosP * pt, pyin = sinP * pt;
+  p[0] = p[0] + k * (pxin * sina - pyin * omca);
+  p[1] = p[1] + k * (pyin * sina + pxin * omca);
+  p[2] = hz;
+  p[4] = p[4] + alpha;
+
+  float sC, cC;
+  sincosf(cosP * sina, &sC, &cC);
+  const float omcC = 1.f - cC;
+  const float sPsC = sinP * sC;
+  const float c2sC = cosP * cosP * sC;
+  const float dzc = deltaZ * cosa;
+  const float iTpt = icosT * pt;
+  const float iT2 = icosT * icosT;
+  const float kpt = k * pt;
+  const float kpt2 = kpt * pt;
+  Jac J;
+  J.J02 = cosP * sinT * cosa * (sPsC - 1.f) * icosT;
+  J.J03 = cosP * sinT * dzc * (1.f - sPsC) * iTpt - kpt2 * (cosP * sina - sinP * omcC);
+  J.J04 = kpt * (sinP * sina * (sPsC - 1.f) - cosP * omcC);
+  J.J05 = cosP * dzc * (1.f - sPsC) * iT2;
+  J.J12 = cosa * sinT * (c2sC - sinP) * icosT;
+  J.J13 = sinT * dzc * (c2sC + sinP) * iTpt - kpt2 * (sinP * sina + cosP * omcC);
+  J.J14 = kpt * (cosP * sina - sinP * omcC - cosP * sina * sPsC);
+  J.J15 = dzc * (c2sC + sinP) * iT2;
+  J.J42 = -ipt * sinT * icosTk;
+  J.J43 = sinT * deltaZ * icosTk;
+  J.J45 = ipt * deltaZ * (icosT * icosTk);
+  return J;
+}
+
+// P' = Jt P Jt^T.  kZRow0: the input z row/column is known to be zero.
+template <bool kZRow0>
+__device__ inline void propagate_cov_fast(float* P, const Jac& J) {
+#pragma clang fp contract(fast)
+  // rows 0, 1, 4 of T = Jt P (rows 3 and 5 are rows of P; row 2 is zero).
+  // Column 2 of T is only needed when P's z column can be non-zero, and then
+  // only for the (i, j) sums below through T[i][2] * J.j2.
+  float T0[6], T1[6], T4[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    if (kZRow0 && c == 2) {
+      T0[c] = T1[c] = T4[c] = 0.f;
+      continue;
+    }
+    const float p0 = P[S(0, c)], p1 = P[S(1, c)];
+    const float p3 = P[S(3, c)], p4 = P[S(4, c)], p5 = P[S(5, c)];
+    float a = p0 + J.J03 * p3 + J.J04 * p4 + J.J05 * p5;
+    float b = p1 + J.J13 * p3 + J.J14 * p4 + J.J15 * p5;
+    float d = p4 + J.J43 * p3 + J.J45 * p5;
+    if (!kZRow0) {
+      const float p2 = P[S(2, c)];
+      a += J.J02 * p2;
+      b += J.J12 * p2;
+      d += J.J42 * p2;
+    }
+    T0[c] = a;
+    T1[c] = b;
+    T4[c] = d;
+  }
+  float T3[6], T5[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    T3[c] = (kZRow0 && c == 2) ? 0.f : P[S(3, c)];
+    T5[c] = (kZRow0 && c == 2) ? 0.f : P[S(5, c)];
+  }
+  // (i, 0): Ti0 + Ti2 J02 + Ti3 J03 + Ti4 J04 + Ti5 J05 ; (i, 1) likewise with
+  // row 1 of Jt; (i, 3) = Ti3; (i, 4) = Ti2 J42 + Ti3 J43 + Ti4 + Ti5 J45.
+#define P2Z_ROW0(T) (T[0] + (kZRow0 ? 0.f : T[2] * J.J02) + T[3] * J.J03 + T[4] * J.J04 + T[5] * J.J05)
+#define P2Z_ROW1(T) (T[1] + (kZRow0 ? 0.f : T[2] * J.J12) + T[3] * J.J13 + T[4] * J.J14 + T[5] * J.J15)
+#define P2Z_ROW4(T) (T[4] + (kZRow0 ? 0.f : T[2] * J.J42) + T[3] * J.J43 + T[5] * J.J45)
+  P[S(0, 0)] = P2Z_ROW0(T0);
+  P[S(1, 0)] = P2Z_ROW0(T1);
+  P[S(1, 1)] = P2Z_ROW1(T1);
+  P[S(3, 0)] = P2Z_ROW0(T3);
+  P[S(3, 1)] = P2Z_ROW1(T3);
+  P[S(3, 3)] = T3[3];
+  P[S(4, 0)] = P2Z_ROW0(T4);
+  P[S(4, 1)] = P2Z_ROW1(T4);
+  P[S(4, 3)] = T4[3];
+  P[S(4, 4)] = P2Z_ROW4(T4);
+  P[S(5, 0)] = P2Z_ROW0(T5);
+  P[S(5, 1)] = P2Z_ROW1(T5);
+  P[S(5, 3)] = T5[3];
+  P[S(5, 4)] = P2Z_ROW4(T5);
+  P[S(5, 5)] = T5[5];
+#undef P2Z_ROW0
+#undef P2Z_ROW1
+#undef P2Z_ROW4
+  P[S(2, 0)] = 0.f;
+  P[S(2, 1)] = 0.f;
+  P[S(2, 2)] = 0.f;
+  P[S(3, 2)] = 0.f;
+  P[S(4, 2)] = 0.f;
+  P[S(5, 2)] = 0.f;
+}
+
+template <bool kZRow0>
+__device__ inline void propagate_fast(Track& t, float hz) {
+  const Jac J = helix_fast(t, hz);
+  propagate_cov_fast<kZRow0>(t.P, J);
+}
+
+// Rows of the state that carry information.  After a propagation row 2 is
+// zero and the update skips it (kZRow0); a standalone update of an arbitrary
+// covariance uses all six rows.
+template <bool kZRow0>
+struct Live {
+  static constexpr int n = kZRow0 ? 5 : 6;
+  __device__ static constexpr int at(int a) { return kZRow0 ? (a < 2 ? a : a + 1) : a; }
+};
+
+template <bool kZRow0>
+__device__ inline void update_fast(Track& t, const Hit& h) {
+#pragma clang fp contract(fast)
+  using L = Live<kZRow0>;
+  float* p = t.p;
+  float* P = t.P;
+  const float r00 = P[0] + h.xx;
+  const float r01 = P[1] + h.xy;
+  const float r11 = P[2] + h.yy;
+  const float s = fast_rcp(r00 * r11 - r01 * r01);
+  const float i00 = s * r11, i01 = -s * r01, i11 = s * r00;
+  float K0[6], K1[6];
+#pragma unroll
+  for (int n = 0; n < L::n; ++n) {
+    const int i = L::at(n);
+    const float a = P[S(i, 0)], b = P[S(i, 1)];
+    K0[i] = a * i00 + b * i01;
+    K1[i] = a * i01 + b * i11;
+  }
+  const float res0 = h.x - p[0];
+  const float res1 = h.y - p[1];
+#pragma unroll
+  for (int n = 0; n < L::n; ++n) {
+    const int i = L::at(n);
+    p[i] = p[i] + K0[i] * res0 + K1[i] * res1;
+  }
+  // P - K (H P); H P = rows 0 and 1 of P.
+  float HP0[6], HP1[6];
+#pragma unroll
+  for (int n = 0; n < L::n; ++n) {
+    const int j = L::at(n);
+    HP0[j] = P[S(0, j)];
+    HP1[j] = P[S(1, j)];
+  }
+#pragma unroll
+  for (int a = 0; a < L::n; ++a)
+#pragma unroll
+    for (int b = 0; b <= a; ++b) {
+      const int i = L::at(a), j = L::at(b);
+      P[S(i, j)] = P[S(i, j)] - K0[i] * HP0[j] - K1[i] * HP1[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kFp64Gain: FP64 inverse, gain, parameter update and Joseph-form covariance
+// ---------------------------------------------------------------------------
+
+template <bool kZRow0>
+__device__ inline void update_fp64gain(Track& t, const Hit& h) {
+#pragma clang fp contract(fast)
+  using L = Live<kZRow0>;
+  constexpr int n = L::n;
+  float* p = t.p;
+  float* P = t.P;
+  // the live block in double, packed over live indices
+  double Q[21];
+#pragma unroll
+  for (int a = 0; a < n; ++a)
+#pragma unroll
+    for (int b = 0; b <= a; ++b) Q[a * (a + 1) / 2 + b] = (double)P[S(L::at(a), L::at(b))];
+  auto q = [&](int a, int b) -> double { return a >= b ? Q[a * (a + 1) / 2 + b] : Q[b * (b + 1) / 2 + a]; };
+  const double R00 = h.xx, R01 = h.xy, R11 = h.yy;
+  const double r00 = q(0, 0) + R00, r01 = q(1, 0) + R01, r11 = q(1, 1) + R11;
+  const double s = 1.0 / (r00 * r11 - r01 * r01);
+  const double i00 = s * r11, i01 = -s * r01, i11 = s * r00;
+  double K0[6], K1[6];
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    K0[a] = q(a, 0) * i00 + q(a, 1) * i01;
+    K1[a] = q(a, 0) * i01 + q(a, 1) * i11;
+  }
+  const double res0 = (double)h.x - (double)p[0];
+  const double res1 = (double)h.y - (double)p[1];
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    const int i = L::at(a);
+    p[i] = (float)((double)p[i] + K0[a] * res0 + K1[a] * res1);
+  }
+  // Joseph form: M = (I - K H) Q ; P' = M (I - K H)^T + K R K^T
+  double M[6][6];
+#pragma unroll
+  for (int a = 0; a < n; ++a)
+#pragma unroll
+    for (int b = 0; b < n; ++b) M[a][b] = q(a, b) - K0[a] * q(0, b) - K1[a] * q(1, b);
+  double KR0[6], KR1[6];
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    KR0[a] = K0[a] * R00 + K1[a] * R01;
+    KR1[a] = K0[a] * R01 + K1[a] * R11;
+  }
+#pragma unroll
+  for (int a = 0; a < n; ++a)
+#pragma unroll
+    for (int b = 0; b <= a; ++b) {
+      const double v = M[a][b] - M[a][0] * K0[b] - M[a][1] * K1[b] + KR0[a] * K0[b] + KR1[a] * K1[b];
+      P[S(L::at(a), L::at(b))] = (float)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one layer
+// ---------------------------------------------------------------------------
+
+// kFirst: the covariance may still have a non-zero z row (first layer of a
+// track); otherwise the fast forms drop the structurally-zero terms.
+template <int kMath, bool kFirst>
+__device__ inline void layer_step(Track& t, const Hit& h) {
+  if constexpr (kMath == kRef) {
+    propagate_ref(t, h.z);
+    update_ref(t, h);
+  } else if constexpr (kMath == kFast) {
+    propagate_fast<!kFirst>(t, h.z);
+    update_fast<true>(t, h);
+  } else {
+    propagate_fast<!kFirst>(t, h.z);
+    update_fp64gain<true>(t, h);
+  }
+}
+
+}  // namespace p2z

@@ -1,0 +1,358 @@
+"""ctypes binding of libp2z.so, the C ABI declared in include/p2z.h.
+
+Host arrays are numpy arrays in the reference's AoSoA layout (float32 words;
+the charge lanes hold int32 bit patterns):
+    MPTRK block: par[6W] | cov[21W] | q[W]   (28W words)
+    MPHIT block: pos[3W] | cov[6W]           ( 9W words)
+Device memory is owned by DeviceBuffer objects allocated through the library.
+
+The library must be built (``python -c "import __graft_entry__ as g; g.build()"``
+at the repository root); loading fails loudly otherwise.  There is no CPU
+fallback: every compute call runs on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libp2z.so")
+MAIN_PATH = os.path.join(HERE, "bin", "p2z_main")
+
+TRK_WORDS = 28
+HIT_WORDS = 9
+WIDTHS = (16, 32, 64, 128)
+MATH = {"fast": 0, "ref": 1, "fp64gain": 2}
+PRESET = {"ref": 0, "config0": 1}
+
+OK, EINVAL, ENOMEM, EHIP, ENODEV, EBUSY, ESTATE = 0, -1, -2, -3, -4, -5, -6
+
+
+class P2zError(RuntimeError):
+    def __init__(self, rc: int, what: str, msg: str):
+        super().__init__(f"{what} failed (rc={rc}): {msg}")
+        self.rc = rc
+
+
+class ATRK(C.Structure):
+    _fields_ = [("par", C.c_float * 6), ("cov", C.c_float * 21), ("q", C.c_int32)]
+
+
+class AHIT(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("cov", C.c_float * 6)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("tracks_per_wg", C.c_int32), ("nstreams", C.c_int32),
+                ("hit_stage", C.c_int32), ("reserved", C.c_int32)]
+
+
+SUMMARY_FIELDS = ("avgx avgy avgz avgdx avgdy avgdz avgpt avgphi avgtheta "
+                  "stdx stdy stdz stddx stddy stddz").split()
+
+
+class Summary(C.Structure):
+    _fields_ = [(n, C.c_double) for n in SUMMARY_FIELDS] + [
+        ("nnans", C.c_int32), ("nfail", C.c_int32), ("ngood", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        d = {n: getattr(self, n) for n in SUMMARY_FIELDS}
+        d.update(nnans=self.nnans, nfail=self.nfail, ngood=self.ngood)
+        return d
+
+
+_lib = None
+vp, i32, i64, u32, dbl, sz = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_double, C.c_size_t
+
+_SIGS = {
+    "p2z_version": ([], i32),
+    "p2z_last_error": ([], C.c_char_p),
+    "p2z_device_count": ([C.POINTER(C.c_int)], i32),
+    "p2z_create": ([C.POINTER(vp), C.c_int, C.POINTER(Opts)], i32),
+    "p2z_destroy": ([vp], i32),
+    "p2z_sync": ([vp], i32),
+    "p2z_set_opts": ([vp, C.POINTER(Opts)], i32),
+    "p2z_get_stream": ([vp, C.POINTER(vp)], i32),
+    "p2z_dev_alloc": ([vp, C.POINTER(vp), sz], i32),
+    "p2z_dev_free": ([vp, vp], i32),
+    "p2z_host_alloc": ([vp, C.POINTER(vp), sz], i32),
+    "p2z_host_free": ([vp, vp], i32),
+    "p2z_memcpy_h2d": ([vp, vp, vp, sz], i32),
+    "p2z_memcpy_d2h": ([vp, vp, vp, sz], i32),
+    "p2z_timer_start": ([vp], i32),
+    "p2z_timer_stop": ([vp, C.POINTER(C.c_float)], i32),
+    "p2z_srand": ([C.c_uint], None),
+    "p2z_randn": ([C.c_float, C.c_float], C.c_float),
+    "p2z_prepare_tracks": ([C.POINTER(ATRK), vp, i64, i64, i32, dbl], i32),
+    "p2z_prepare_hits": ([C.POINTER(AHIT), i32, vp, i64, i64, i32, i32, dbl], i32),
+    "p2z_preset_input": ([i32, C.POINTER(ATRK), C.POINTER(AHIT), i32, C.POINTER(dbl)], i32),
+    "p2z_propagate_to_z": ([vp, vp, vp, vp, vp, vp, vp, i64, i64, i64, i32, i32], i32),
+    "p2z_kalman_update": ([vp, vp, vp, vp, vp, i64, i64, i64, i32, i32], i32),
+    "p2z_run": ([vp, vp, vp, vp, i64, i64, i32, i32, i32], i32),
+    "p2z_run_host": ([vp, vp, vp, vp, i64, i64, i32, i32, i32], i32),
+    "p2z_summary_device": ([vp, vp, i64, i64, i32, i64, C.POINTER(C.c_float), C.POINTER(Summary)], i32),
+    "p2z_summary_host": ([vp, i64, i64, i32, i64, C.POINTER(C.c_float), C.POINTER(Summary)], i32),
+    "p2z_format_summary": ([C.POINTER(Summary), C.c_char_p, i32], i32),
+}
+
+
+def lib():
+    """Loads libp2z.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() at the repository root")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def last_error() -> str:
+    return lib().p2z_last_error().decode()
+
+
+def check(rc: int, what: str) -> int:
+    if rc < 0:
+        raise P2zError(rc, what, last_error())
+    return rc
+
+
+def _ptr(a) -> int:
+    if isinstance(a, DeviceBuffer):
+        return a.ptr
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
+        return a.ctypes.data
+    if isinstance(a, int):
+        return a
+    raise TypeError(f"cannot pass {type(a)} as a buffer")
+
+
+# ---------------------------------------------------------------------------
+# host-side inputs
+# ---------------------------------------------------------------------------
+
+def trk_words(nblocks: int, width: int) -> int:
+    return nblocks * TRK_WORDS * width
+
+
+def hit_words(nblocks: int, nlayer: int, width: int) -> int:
+    return nblocks * nlayer * HIT_WORDS * width
+
+
+def preset_input(name: str):
+    """Returns (ATRK, [AHIT]*26, smear) of a preset ('ref' or 'config0')."""
+    t = ATRK()
+    hits = (AHIT * 26)()
+    smear = C.c_double()
+    n = check(lib().p2z_preset_input(PRESET[name], C.byref(t), hits, 26, C.byref(smear)), "p2z_preset_input")
+    return t, list(hits)[:n], smear.value
+
+
+def atrk_from_array(a) -> ATRK:
+    """ATRK from 28 numbers: par[6], cov[21], q."""
+    t = ATRK()
+    for i in range(6):
+        t.par[i] = float(a[i])
+    for i in range(21):
+        t.cov[i] = float(a[6 + i])
+    t.q = int(a[27])
+    return t
+
+
+def ahits_from_array(a) -> list:
+    a = np.asarray(a, dtype=np.float64).reshape(-1, 9)
+    hits = []
+    for row in a:
+        h = AHIT()
+        for i in range(3):
+            h.pos[i] = float(row[i])
+        for i in range(6):
+            h.cov[i] = float(row[3 + i])
+        hits.append(h)
+    return hits
+
+
+def srand(seed: int) -> None:
+    lib().p2z_srand(seed)
+
+
+def prepare_tracks(trk: ATRK, nevts: int, nb: int, width: int, smear: float) -> np.ndarray:
+    """prepareTracks (OMP:243-263) on the current rand() stream."""
+    out = np.empty(trk_words(nevts * nb, width), np.float32)
+    check(lib().p2z_prepare_tracks(C.byref(trk), out.ctypes.data, nevts, nb, width, smear), "p2z_prepare_tracks")
+    return out
+
+
+def prepare_hits(hits: list, nevts: int, nb: int, width: int, nlayer: int, smear: float) -> np.ndarray:
+    """prepareHits (OMP:265-293) on the current rand() stream."""
+    arr = (AHIT * len(hits))(*hits)
+    out = np.empty(hit_words(nevts * nb, nlayer, width), np.float32)
+    check(lib().p2z_prepare_hits(arr, len(hits), out.ctypes.data, nevts, nb, width, nlayer, smear),
+          "p2z_prepare_hits")
+    return out
+
+
+def generate(preset: str, nevts: int, ntrks: int, width: int, nlayer: int, seed: int = 1):
+    """srand(seed) + prepareTracks + prepareHits as the reference's main does
+    (OMP:715-720).  Returns (trk, hit, lasthit[3], nb)."""
+    t, hits, smear = preset_input(preset)
+    nb = ntrks // width
+    srand(seed)
+    trk = prepare_tracks(t, nevts, nb, width, smear)
+    hit = prepare_hits(hits, nevts, nb, width, nlayer, smear)
+    last = np.array(list(hits[nlayer - 1].pos), np.float32)
+    return trk, hit, last, nb
+
+
+def summary_host(out: np.ndarray, nevts: int, nb: int, width: int, ntrks: int, lasthit) -> Summary:
+    s = Summary()
+    last = (C.c_float * 3)(*[float(v) for v in lasthit])
+    check(lib().p2z_summary_host(out.ctypes.data, nevts, nb, width, ntrks, last, C.byref(s)), "p2z_summary_host")
+    return s
+
+
+def format_summary(s: Summary) -> str:
+    buf = C.create_string_buffer(4096)
+    check(lib().p2z_format_summary(C.byref(s), buf, 4096), "p2z_format_summary")
+    return buf.value.decode()
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().p2z_device_count(C.byref(n)), "p2z_device_count")
+    return n.value
+
+
+# ---------------------------------------------------------------------------
+# device side
+# ---------------------------------------------------------------------------
+
+@dataclass
+class DeviceBuffer:
+    ctx: "Context"
+    ptr: int
+    nbytes: int
+
+    def free(self) -> None:
+        if self.ptr:
+            check(lib().p2z_dev_free(self.ctx.handle, self.ptr), "p2z_dev_free")
+            self.ptr = 0
+
+
+class Context:
+    """A p2z context: one GPU, its streams and buffers."""
+
+    def __init__(self, device: int = 0, tracks_per_wg: int = 0, nstreams: int = 0, hit_stage: int = 0):
+        self.handle = vp()
+        self._opts = Opts(tracks_per_wg, nstreams, hit_stage, 0)
+        check(lib().p2z_create(C.byref(self.handle), device, C.byref(self._opts)), "p2z_create")
+        self._bufs: list[DeviceBuffer] = []
+
+    # lifetime
+    def close(self) -> None:
+        if self.handle:
+            for b in self._bufs:
+                if b.ptr:
+                    lib().p2z_dev_free(self.handle, b.ptr)
+                    b.ptr = 0
+            check(lib().p2z_destroy(self.handle), "p2z_destroy")
+            self.handle = vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_opts(self, tracks_per_wg: int = 0, nstreams: int = 0, hit_stage: int = 0) -> None:
+        self._opts = Opts(tracks_per_wg, nstreams, hit_stage, 0)
+        check(lib().p2z_set_opts(self.handle, C.byref(self._opts)), "p2z_set_opts")
+
+    def sync(self) -> None:
+        check(lib().p2z_sync(self.handle), "p2z_sync")
+
+    # memory
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        p = vp()
+        check(lib().p2z_dev_alloc(self.handle, C.byref(p), nbytes), "p2z_dev_alloc")
+        b = DeviceBuffer(self, p.value or 0, nbytes)
+        self._bufs.append(b)
+        return b
+
+    def upload(self, a: np.ndarray) -> DeviceBuffer:
+        b = self.alloc(a.nbytes)
+        self.h2d(b, a)
+        return b
+
+    def h2d(self, dst: DeviceBuffer, a: np.ndarray) -> None:
+        check(lib().p2z_memcpy_h2d(self.handle, dst.ptr, _ptr(a), a.nbytes), "p2z_memcpy_h2d")
+        self.sync()
+
+    def d2h(self, a: np.ndarray, src: DeviceBuffer) -> np.ndarray:
+        check(lib().p2z_memcpy_d2h(self.handle, _ptr(a), src.ptr, a.nbytes), "p2z_memcpy_d2h")
+        self.sync()
+        return a
+
+    def download(self, src: DeviceBuffer, dtype=np.float32) -> np.ndarray:
+        a = np.empty(src.nbytes // np.dtype(dtype).itemsize, dtype)
+        return self.d2h(a, src)
+
+    # timing (hipEvents on the context's stream)
+    def timer_start(self) -> None:
+        check(lib().p2z_timer_start(self.handle), "p2z_timer_start")
+
+    def timer_stop(self) -> float:
+        ms = C.c_float()
+        check(lib().p2z_timer_stop(self.handle, C.byref(ms)), "p2z_timer_stop")
+        return ms.value
+
+    # the path
+    def run(self, trk, hit, out, nevts: int, nb: int, nlayer: int, width: int, math="fast") -> None:
+        check(lib().p2z_run(self.handle, _ptr(trk), _ptr(hit), _ptr(out), nevts, nb, nlayer, width,
+                            MATH.get(math, math)), "p2z_run")
+
+    def run_host(self, trk: np.ndarray, hit: np.ndarray, out: np.ndarray, nevts: int, nb: int, nlayer: int,
+                 width: int, math="fast") -> None:
+        check(lib().p2z_run_host(self.handle, _ptr(trk), _ptr(hit), _ptr(out), nevts, nb, nlayer, width,
+                                 MATH.get(math, math)), "p2z_run_host")
+
+    def propagate_to_z(self, trk: DeviceBuffer, hit: DeviceBuffer, out: DeviceBuffer, nblocks: int,
+                       width: int, math="fast", hit_stride: int | None = None) -> None:
+        """propagateToZ on MPTRK blocks trk -> out with one MPHIT block per
+        track block (hit block b at b*hit_stride words)."""
+        W = width
+        hs = HIT_WORDS * W if hit_stride is None else hit_stride
+        f4 = 4
+        check(lib().p2z_propagate_to_z(self.handle, trk.ptr + 6 * W * f4, trk.ptr, trk.ptr + 27 * W * f4, hit.ptr,
+                                       out.ptr + 6 * W * f4, out.ptr, nblocks, TRK_WORDS * W, hs, W,
+                                       MATH.get(math, math)), "p2z_propagate_to_z")
+
+    def kalman_update(self, trk: DeviceBuffer, hit: DeviceBuffer, nblocks: int, width: int, math="fast",
+                      hit_stride: int | None = None) -> None:
+        """KalmanUpdate_v2 in place on MPTRK blocks with one MPHIT block each."""
+        W = width
+        hs = HIT_WORDS * W if hit_stride is None else hit_stride
+        f4 = 4
+        check(lib().p2z_kalman_update(self.handle, trk.ptr + 6 * W * f4, trk.ptr, hit.ptr + 3 * W * f4, hit.ptr,
+                                      nblocks, TRK_WORDS * W, hs, W, MATH.get(math, math)), "p2z_kalman_update")
+
+    def summary_device(self, out: DeviceBuffer, nevts: int, nb: int, width: int, ntrks: int, lasthit) -> Summary:
+        s = Summary()
+        last = (C.c_float * 3)(*[float(v) for v in lasthit])
+        check(lib().p2z_summary_device(self.handle, out.ptr, nevts, nb, width, ntrks, last, C.byref(s)),
+              "p2z_summary_device")
+        return s

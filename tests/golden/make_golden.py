@@ -1,0 +1,103 @@
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+The reference (src/propagate-toz-test_OMP.cpp of cerati/p2z-tests) is compiled
+from its own source by `make -C oracle ref` into oracle/_ref/, once per size
+configuration, with -ffp-contract=off.  This script runs those builds and
+stores their inputs and outputs as small data files, so the parity tests can
+check the oracle and the GPU path where the reference tree is absent.
+
+    python tests/golden/make_golden.py          (needs oracle/_ref built)
+
+Files:
+  ref_input.json        the reference's input track and 26 hits in main's
+                        layer order (src/input_track.h, OMP:693-695)
+  known_answer.txt      the 11 summary lines of logs/propagate_gcc_omp.log:30-40
+  ref_small.npz         nevts=3 ntrks=96 bsize=32 nlayer=20: inputs and one
+                        driver pass of outputs (every track)
+  ref_w16.npz           nevts=2 ntrks=64 bsize=16 nlayer=20: same
+  ref_track_layers.npz  nevts=1 ntrks=1 bsize=1: the state after each of the
+                        20 layers, after propagateToZ and after KalmanUpdate_v2
+  ref_summary_full.txt  the reference's own report at the full configuration
+                        (nevts=100 ntrks=9600 nlayer=20), contraction off
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+import pyoracle  # noqa: E402
+
+REF_LOG = "/root/reference/logs/propagate_gcc_omp.log"
+
+
+def need(name):
+    L = pyoracle.ref_lib(name)
+    if L is None:
+        sys.exit(f"oracle/_ref/libref_{name}.so missing: run `make -C oracle ref` first")
+    return L
+
+
+def main() -> None:
+    L = need("small")
+    t, h = pyoracle.ref_inputs(L)
+    with open(os.path.join(HERE, "ref_input.json"), "w") as f:
+        json.dump({"source": "src/input_track.h:1-113 in main's layer order (OMP:693-695)",
+                   "track": [float(v) for v in t], "hits": [[float(v) for v in row] for row in h],
+                   "smear": 1e-7, "seed": 1}, f, indent=1)
+
+    with open(REF_LOG) as f:
+        lines = f.read().splitlines()
+    ka = [ln for ln in lines if (ln.startswith("track ") and not ln.startswith("track in ")) or ln.startswith("number of tracks")]
+    assert len(ka) == 11, ka
+    with open(os.path.join(HERE, "known_answer.txt"), "w") as f:
+        f.write("\n".join(ka) + "\n")
+
+    for name in ("small", "w16"):
+        L = need(name)
+        c = pyoracle.ref_config(L)
+        trk, hit = pyoracle.ref_generate(L)
+        out = pyoracle.ref_run(L, trk, hit)
+        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), trk=trk, hit=hit, out=out,
+                            **{k: np.int64(v) for k, v in c.items()})
+
+    # one track, layer by layer, through the reference's own per-block functions
+    L = need("w1")
+    c = pyoracle.ref_config(L)
+    assert c["width"] == 1 and c["nevts"] == 1 and c["nb"] == 1
+    trk, hit = pyoracle.ref_generate(L)
+    state = trk.copy()
+    after_prop, after_upd = [], []
+    for lay in range(c["nlayer"]):
+        hb = np.ascontiguousarray(hit[lay * 9:(lay + 1) * 9])
+        nxt = np.empty_like(state)
+        L.ref_propagate_block(state.ctypes.data, hb.ctypes.data, nxt.ctypes.data)
+        after_prop.append(nxt.copy())
+        L.ref_update_block(nxt.ctypes.data, hb.ctypes.data)
+        after_upd.append(nxt.copy())
+        state = nxt
+    np.savez_compressed(os.path.join(HERE, "ref_track_layers.npz"), trk=trk, hit=hit,
+                        after_prop=np.array(after_prop), after_update=np.array(after_upd))
+
+    # the reference's own report at full size (runs its main)
+    full = os.path.join(REPO, "oracle", "_ref", "libref_full.so")
+    code = f"import ctypes; ctypes.CDLL({full!r}).ref_main()"
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True,
+                         env=dict(os.environ, OMP_NUM_THREADS="8"))
+    rep = [ln for ln in res.stdout.splitlines() if ln.startswith("track ") or ln.startswith("number of")]
+    rep = [ln for ln in rep if not ln.startswith("track in ")]
+    assert len(rep) == 11, res.stdout
+    with open(os.path.join(HERE, "ref_summary_full.txt"), "w") as f:
+        f.write("\n".join(rep) + "\n")
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,57 @@
+"""Parity metric shared by the GPU tests, smoke() and the precision report.
+
+Per track, relative differences against a reference result b:
+  parameters : |a - b| / max(|b|, 1e-6)
+  covariance : |a - b| / max(|b|, floor_ij), floor_ij = COV_FLOOR * sqrt(|b_ii b_jj|)
+The covariance floor is a fraction of the entry's correlation scale, so that
+entries the update drives to (near) zero by cancellation are measured
+against the size of the matrix they belong to, and exact zeros compare equal.
+Tracks whose parameters are not finite in b must be non-finite in a too and
+are excluded from the relative measures.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+PAR_TOL = 1e-5
+COV_TOL = 1e-4
+COV_FLOOR = 1e-3
+_DIAG = [0, 2, 5, 9, 14, 20]
+_IJ = [(i, j) for i in range(6) for j in range(i + 1)]
+
+
+def track_major(blocks: np.ndarray, width: int) -> np.ndarray:
+    """MPTRK blocks (flat float32) -> (ntracks, 28) float64; column 27 = q."""
+    nb = blocks.size // (28 * width)
+    t = blocks.reshape(nb, 28, width).transpose(0, 2, 1).reshape(-1, 28)
+    out = t.astype(np.float64)
+    out[:, 27] = t[:, 27].copy().view(np.int32)
+    return out
+
+
+def compare(a: np.ndarray, b: np.ndarray) -> dict:
+    """a, b: (ntracks, 28) float64 (track_major or the FP64 oracle's output)."""
+    fin_b = np.all(np.isfinite(b[:, :6]), axis=1)
+    fin_a = np.all(np.isfinite(a[:, :6]), axis=1)
+    a, b = a[fin_b], b[fin_b]
+    par = np.abs(a[:, :6] - b[:, :6]) / np.maximum(np.abs(b[:, :6]), 1e-6)
+    Pb = b[:, 6:27]
+    d = np.abs(Pb[:, _DIAG])
+    scale = np.stack([np.sqrt(d[:, _DIAG.index(_DIAG[i])] * d[:, _DIAG.index(_DIAG[j])]) for i, j in _IJ], 1)
+    cov = np.abs(a[:, 6:27] - Pb) / np.maximum(np.maximum(np.abs(Pb), COV_FLOOR * scale), 1e-300)
+    return dict(
+        ntracks=int(fin_b.size),
+        nonfinite_ref=int((~fin_b).sum()),
+        nonfinite_mismatch=int((fin_a != fin_b).sum()),
+        charge_mismatch=int((a[:, 27] != b[:, 27]).sum()),
+        par_max=float(par.max()) if par.size else 0.0,
+        par_median=float(np.median(par)) if par.size else 0.0,
+        cov_max=float(cov.max()) if cov.size else 0.0,
+        cov_median=float(np.median(cov)) if cov.size else 0.0,
+        par_max_by_elem=[float(v) for v in par.max(axis=0)] if par.size else [],
+    )
+
+
+def ok(stats: dict, par_tol: float = PAR_TOL, cov_tol: float = COV_TOL) -> bool:
+    return (stats["nonfinite_mismatch"] == 0 and stats["charge_mismatch"] == 0
+            and stats["par_max"] <= par_tol and stats["cov_max"] <= cov_tol)
