@@ -35,6 +35,18 @@
 #include <omp.h>
 #endif
 
+/* sin / cos of the path.  Default: the C library's sinf / cosf, as the
+ * reference calls them.  With -DORACLE_CR_LIBM: correctly rounded values
+ * (evaluated in double, then rounded), i.e. the same algorithm under a
+ * different libm; the pair measures the path's sensitivity to the libm. */
+#ifdef ORACLE_CR_LIBM
+static inline float path_sinf(float x) { return (float)sin((double)x); }
+static inline float path_cosf(float x) { return (float)cos((double)x); }
+#else
+#define path_sinf sinf
+#define path_cosf cosf
+#endif
+
 /* kfact as the reference declares it: a float initialised from a double
  * expression (OMP:622). */
 static const float kKfact = 100. / (-0.299792458 * 3.8112);
@@ -125,17 +137,17 @@ static void lane_propagate(float P[21], float par[6], int q, float hz) {
   const float deltaZ = zout - par[2];
   const float ipt = par[3];
   const float pt = (float)(1. / ipt);                 /* double divide   */
-  const float cosP = cosf(par[4]);
-  const float sinP = sinf(par[4]);
-  const float cosT = cosf(par[5]);
-  const float sinT = sinf(par[5]);
+  const float cosP = path_cosf(par[4]);
+  const float sinP = path_sinf(par[4]);
+  const float cosT = path_cosf(par[5]);
+  const float sinT = path_sinf(par[5]);
   const float pxin = cosP * pt;
   const float pyin = sinP * pt;
   const float icosT = (float)(1.0 / cosT);            /* double divide   */
   const float icosTk = icosT / k;                     /* float divide    */
   const float alpha = deltaZ * sinT * ipt * icosTk;
-  const float sina = sinf(alpha);
-  const float cosa = cosf(alpha);
+  const float sina = path_sinf(alpha);
+  const float cosa = path_cosf(alpha);
 
   /* x, y: the (1.-cosa) factor promotes the rest of the expression to double */
   par[0] = (float)(par[0] + k * ((double)(pxin * sina) - pyin * (1. - cosa)));
@@ -144,8 +156,8 @@ static void lane_propagate(float P[21], float par[6], int q, float hz) {
   /* par[3] (ipt) and par[5] (theta) unchanged */
   par[4] = par[4] + alpha;
 
-  const float sCosPsina = sinf(cosP * sina);
-  const float cCosPsina = cosf(cosP * sina);
+  const float sCosPsina = path_sinf(cosP * sina);
+  const float cCosPsina = path_cosf(cosP * sina);
 
   /* the eleven off-diagonal Jacobian entries (OMP:656-666) */
   const float J02 = cosP * sinT * (sinP * cosa * sCosPsina - cosa) * icosT;

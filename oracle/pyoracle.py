@@ -14,6 +14,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "liboracle_p2z.so")
+# the same oracle with correctly rounded sin/cos in the path
+ORACLE_LIB_CR = os.path.join(HERE, "liboracle_p2z_crlibm.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 TRK_WORDS, HIT_WORDS = 28, 9
@@ -56,21 +58,40 @@ _SIGS = {
     "oracle_format_summary": ([C.POINTER(OSummary), C.c_char_p, C.c_int], C.c_int),
 }
 
-_lib = None
+_libs: dict = {}
+_variant = ["glibc"]
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_LIB):
-            raise FileNotFoundError(f"{ORACLE_LIB} is not built (make -C oracle)")
-        L = C.CDLL(ORACLE_LIB)
+def lib(variant: str | None = None):
+    """The oracle library: variant 'glibc' (sinf/cosf of the C library, as the
+    reference) or 'cr' (correctly rounded sin/cos).  Default: the current
+    variant (see use_variant)."""
+    v = variant or _variant[0]
+    if v not in _libs:
+        path = ORACLE_LIB if v == "glibc" else ORACLE_LIB_CR
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (make -C oracle)")
+        L = C.CDLL(path)
         for n, (a, r) in _SIGS.items():
             f = getattr(L, n)
             f.argtypes = a
             f.restype = r
-        _lib = L
-    return _lib
+        _libs[v] = L
+    return _libs[v]
+
+
+class use_variant:
+    """with use_variant('cr'): ... runs the oracle functions on that variant."""
+
+    def __init__(self, v: str):
+        self.v = v
+
+    def __enter__(self):
+        self.prev = _variant[0]
+        _variant[0] = self.v
+
+    def __exit__(self, *exc):
+        _variant[0] = self.prev
 
 
 def _p(a: np.ndarray) -> int:

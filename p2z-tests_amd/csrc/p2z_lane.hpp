@@ -9,7 +9,12 @@
 //               MultHelixPropTranspEndcap 343-373, KalmanUpdate_v2 503-620):
 //               FP32 with the FP64 sub-expressions the reference's `1.`
 //               literals and `(double)` casts create, no contraction, the
-//               reference's summation order.
+//               reference's summation order, and correctly rounded sin/cos
+//               (evaluated in FP64, then rounded).  It reproduces the
+//               reference algorithm bit for bit under a correctly rounded
+//               libm; against glibc's sinf/cosf, which are not correctly
+//               rounded everywhere, it differs as the reference does when its
+//               own libm or FP model changes.
 //   kFast     : FP32 with FMA contraction, native reciprocals and the sparse
 //               structure of the Jacobian and of the covariance exploited.
 //   kFp64Gain : kFast propagation; the 2x2 residual-covariance inverse, the
@@ -55,6 +60,16 @@ struct Hit {
 // kRef: reference typing
 // ---------------------------------------------------------------------------
 
+// Correctly rounded single-precision sin and cos: the FP64 results are
+// within an FP64 ulp, so rounding them to FP32 gives the nearest float except
+// in cases of probability ~2^-28.
+__device__ inline void sincos_cr(float x, float* s, float* c) {
+  double sd, cd;
+  sincos((double)x, &sd, &cd);
+  *s = (float)sd;
+  *c = (float)cd;
+}
+
 __device__ inline void propagate_ref(Track& t, float hz) {
 #pragma clang fp contract(off)
   float* p = t.p;
@@ -65,22 +80,22 @@ __device__ inline void propagate_ref(Track& t, float hz) {
   const float ipt = p[3];
   const float pt = (float)(1. / (double)ipt);
   float sinP, cosP, sinT, cosT;
-  sincosf(p[4], &sinP, &cosP);
-  sincosf(p[5], &sinT, &cosT);
+  sincos_cr(p[4], &sinP, &cosP);
+  sincos_cr(p[5], &sinT, &cosT);
   const float pxin = cosP * pt;
   const float pyin = sinP * pt;
   const float icosT = (float)(1.0 / (double)cosT);
   const float icosTk = icosT / k;
   const float alpha = deltaZ * sinT * ipt * icosTk;
   float sina, cosa;
-  sincosf(alpha, &sina, &cosa);
+  sincos_cr(alpha, &sina, &cosa);
   p[0] = (float)((double)p[0] + (double)k * ((double)(pxin * sina) - (double)pyin * (1. - (double)cosa)));
   p[1] = (float)((double)p[1] + (double)k * ((double)(pyin * sina) + (double)pxin * (1. - (double)cosa)));
   p[2] = zout;
   p[4] = p[4] + alpha;
 
   float sC, cC;
-  sincosf(cosP * sina, &sC, &cC);
+  sincos_cr(cosP * sina, &sC, &cC);
   const double omcC = 1. - (double)cC;
   const float J02 = cosP * sinT * (sinP * cosa * sC - cosa) * icosT;
   const float J03 = (float)((double)(cosP * sinT * deltaZ * cosa) * (1. - (double)(sinP * sC)) * (double)(icosT * pt)

@@ -19,6 +19,19 @@ Files:
                         20 layers, after propagateToZ and after KalmanUpdate_v2
   ref_summary_full.txt  the reference's own report at the full configuration
                         (nevts=100 ntrks=9600 nlayer=20), contraction off
+  ref_steps.npz         8 random blocks (bsize 32, seeded): one propagateToZ and
+                        one KalmanUpdate_v2 each, from both reference builds
+  ref_spread.json       the reference's own per-track spread on the driver
+                        outputs (small, w16 and mid = 10 events x 960 tracks)
+                        and on the single steps, under
+                          <name>      : its Makefile's FP model (FMA build)
+                                        against the contraction-off build
+                          <name>_libm : the same algorithm with correctly
+                                        rounded sin/cos (the oracle's 'cr'
+                                        variant) against glibc's sinf/cosf
+
+Every *.npz holding driver outputs has `out` (contraction off) and `out_fma`
+(the reference's default FP model, FMA contraction).
 """
 from __future__ import annotations
 
@@ -33,7 +46,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(REPO, "oracle"))
+sys.path.insert(0, os.path.join(REPO, "tests"))
 import pyoracle  # noqa: E402
+from parity import compare, random_blocks, track_major  # noqa: E402
 
 REF_LOG = "/root/reference/logs/propagate_gcc_omp.log"
 
@@ -60,13 +75,60 @@ def main() -> None:
     with open(os.path.join(HERE, "known_answer.txt"), "w") as f:
         f.write("\n".join(ka) + "\n")
 
-    for name in ("small", "w16"):
-        L = need(name)
+    spread = {}
+    for name in ("small", "w16", "mid"):
+        L, Lf = need(name), need(name + "_fma")
         c = pyoracle.ref_config(L)
+        assert c == pyoracle.ref_config(Lf)
         trk, hit = pyoracle.ref_generate(L)
+        trk_f, hit_f = pyoracle.ref_generate(Lf)
         out = pyoracle.ref_run(L, trk, hit)
-        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), trk=trk, hit=hit, out=out,
-                            **{k: np.int64(v) for k, v in c.items()})
+        # the FMA build driven on the contraction-off build's inputs
+        out_fma = pyoracle.ref_run(Lf, trk, hit)
+        W = c["width"]
+        spread[name] = compare(track_major(out_fma, W), track_major(out, W))
+        with pyoracle.use_variant("cr"):
+            out_cr = pyoracle.run(trk, hit, c["nevts"], c["nb"], W, c["nlayer"])
+        spread[name + "_libm"] = compare(track_major(out_cr, W), track_major(out, W))
+        spread[name]["inputs_bitwise_equal_across_builds"] = bool(
+            np.array_equal(trk.view(np.uint32), trk_f.view(np.uint32))
+            and np.array_equal(hit.view(np.uint32), hit_f.view(np.uint32)))
+        if name != "mid":
+            np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), trk=trk, hit=hit, out=out, out_fma=out_fma,
+                                **{k: np.int64(v) for k, v in c.items()})
+
+    # single steps on random blocks through both builds
+    L, Lf = need("small"), need("small_fma")
+    W = L.ref_bsize()
+    rng = np.random.default_rng(2024)
+    blocks = [random_blocks(rng, W, zrow=bool(i % 2)) for i in range(8)]
+    trk = np.concatenate([b[0] for b in blocks])
+    hit = np.concatenate([b[1] for b in blocks])
+    res = {}
+    for tag, lib in (("", L), ("_fma", Lf)):
+        prop, upd = np.empty_like(trk), trk.copy()
+        for b in range(len(blocks)):
+            t = slice(b * 28 * W, (b + 1) * 28 * W)
+            h = slice(b * 9 * W, (b + 1) * 9 * W)
+            tb, hb = np.ascontiguousarray(trk[t]), np.ascontiguousarray(hit[h])
+            pb = np.empty_like(tb)
+            lib.ref_propagate_block(tb.ctypes.data, hb.ctypes.data, pb.ctypes.data)
+            prop[t] = pb
+            ub = tb.copy()
+            lib.ref_update_block(ub.ctypes.data, hb.ctypes.data)
+            upd[t] = ub
+        res["prop" + tag], res["upd" + tag] = prop, upd
+    np.savez_compressed(os.path.join(HERE, "ref_steps.npz"), trk=trk, hit=hit, width=np.int64(W), **res)
+    spread["step_propagate"] = compare(track_major(res["prop_fma"], W), track_major(res["prop"], W))
+    spread["step_update"] = compare(track_major(res["upd_fma"], W), track_major(res["upd"], W))
+    with pyoracle.use_variant("cr"):
+        prop_cr = np.concatenate([pyoracle.propagate_block(np.ascontiguousarray(trk[b * 28 * W:(b + 1) * 28 * W]),
+                                                           np.ascontiguousarray(hit[b * 9 * W:(b + 1) * 9 * W]), W)
+                                  for b in range(len(blocks))])
+    spread["step_propagate_libm"] = compare(track_major(prop_cr, W), track_major(res["prop"], W))
+    spread["step_update_libm"] = compare(track_major(res["upd"], W), track_major(res["upd"], W))
+    with open(os.path.join(HERE, "ref_spread.json"), "w") as f:
+        json.dump(spread, f, indent=1)
 
     # one track, layer by layer, through the reference's own per-block functions
     L = need("w1")

@@ -62,26 +62,8 @@ def test_layer_by_layer_matches_reference(golden):
 
 
 def _random_blocks(rng, W, zrow=True):
-    """Random but physical track blocks: SPD covariance, |cos theta| away from 0."""
-    n = W
-    par = np.stack([rng.uniform(-50, 50, n), rng.uniform(-50, 50, n), rng.uniform(-100, 100, n),
-                    rng.uniform(0.05, 1.0, n) * rng.choice([-1, 1], n), rng.uniform(-3, 3, n),
-                    np.where(rng.random(n) < 0.5, rng.uniform(0.2, 1.2, n), rng.uniform(1.9, 2.9, n))])
-    cov = np.zeros((21, n))
-    for l in range(n):
-        A = rng.normal(size=(6, 6)) * 1e-2
-        S = A @ A.T + np.diag(rng.uniform(1e-5, 1e-3, 6))
-        if not zrow:
-            S[2, :] = S[:, 2] = 0
-        cov[:, l] = [S[i, j] for i in range(6) for j in range(i + 1)]
-    q = rng.choice([-1, 1], n).astype(np.int32)
-    blk = np.concatenate([par.reshape(-1), cov.reshape(-1)]).astype(np.float32)
-    blk = np.concatenate([blk, q.view(np.float32)])
-    hit = np.concatenate([par[0] + rng.normal(0, 0.5, n), par[1] + rng.normal(0, 0.5, n),
-                          par[2] + rng.uniform(-30, 30, n), rng.uniform(1e-4, 1e-1, n),
-                          rng.uniform(-1e-5, 1e-5, n), rng.uniform(1e-4, 1e-1, n),
-                          np.zeros(3 * n)]).astype(np.float32)
-    return blk, hit
+    from parity import random_blocks
+    return random_blocks(rng, W, zrow)
 
 
 def test_random_blocks_match_reference_build():
