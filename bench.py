@@ -62,7 +62,7 @@ def read_traffic(math: str):
 def cpu_baseline(nevts: int, ntrks: int, nlayer: int) -> dict | None:
     """The reference program (src/propagate-toz-test_OMP.cpp, built by
     oracle/Makefile from the reference's own source with its flags) on the
-    same workload with NITER=2 and 16 OpenMP threads; its own timer brackets
+    same workload with NITER=10 (the reference Makefile's value) and 16 OpenMP threads; its own timer brackets
     the NITER region only (OMP:729-753)."""
     cpuflags = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
     has512 = all(f in cpuflags for f in ("avx512f", "avx512bw", "avx512dq", "avx512vl"))

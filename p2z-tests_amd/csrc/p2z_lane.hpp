@@ -192,13 +192,71 @@ __device__ inline void update_ref(Track& t, const Hit& h) {
 // kFast: FP32 + FMA, sparse forms
 // ---------------------------------------------------------------------------
 
-__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// sin and cos of a float, rounded to the nearest float except in ~1e-5 of
+// cases, at about the cost of the library's single-precision sincosf (FP64
+// fused multiply-adds issue at the FP32 rate on CDNA4):
+//   k = rint(x * 2/pi), r = x - k*pi/2 with pi/2 split into two doubles,
+//   sin r = r + r^3 S(r^2) and cos r = 1 + r^2 C(r^2) in FP64 with S, C the
+//   least-squares polynomials of gen_sincos_coeffs.py (max relative error on
+//   |r| <= pi/4: sin 1.916e-14, cos 3.129e-13), then the quadrant k mod 4 selects and
+//   signs the two values.
+// The reduction error is |k| * 1.5e-33 (the next term of pi/2), below the
+// polynomial error for |x| < 2^60; NaN and infinity give NaN.
+// The library's sincosf is faithful only (measured: 10-25% of its results at
+// arguments like the track angles phi ~ 1.4 and theta ~ 2.8 are one ulp from
+// the nearest float); the tracks of an event share nearly the same angles, so
+// those ulps shift every track alike instead of averaging out.
+__device__ inline void sincos_accurate(float xf, float* sp, float* cp) {
+  const double x = (double)xf;
+  const double k = __builtin_rint(x * 0.63661977236758134308);
+  const double r = __builtin_fma(-k, 6.123233995736766036e-17, __builtin_fma(-k, 1.5707963267948965580, x));
+  const double u = r * r;
+  double ps = -2.480613491641723e-08;
+  ps = __builtin_fma(ps, u, 2.755599789925388e-06);
+  ps = __builtin_fma(ps, u, -0.0001984126694941195);
+  ps = __builtin_fma(ps, u, 0.008333333331134977);
+  ps = __builtin_fma(ps, u, -0.16666666666664143);
+  const double sr = __builtin_fma(r * u, ps, r);
+  double pc = -2.7237166798569765e-07;
+  pc = __builtin_fma(pc, u, 2.4799862612840642e-05);
+  pc = __builtin_fma(pc, u, -0.0013888885093552119);
+  pc = __builtin_fma(pc, u, 0.04166666663743458);
+  pc = __builtin_fma(pc, u, -0.4999999999996406);
+  const double cr = __builtin_fma(u, pc, 1.0);
+  // quadrant q = k mod 4; rounding commutes with the swap and the signs
+  const int q = (int)(long long)k & 3;
+  const float sf = (float)sr, cf = (float)cr;
+  const float s = (q & 1) ? cf : sf;
+  const float c = (q & 1) ? sf : cf;
+  *sp = (q & 2) ? -s : s;
+  *cp = ((q + 1) & 2) ? -c : c;
+}
+
+// 1/x rounded to nearest in all but rare cases: the hardware reciprocal
+// (1 ulp) refined by one Newton step with a fused multiply-add.  Every track
+// of an event shares nearly the same ipt / cos(theta); a biased reciprocal
+// would shift all of them the same way, so the 1-ulp estimate alone is not
+// used.
+__device__ inline float fast_rcp(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+
+// a / b through the refined reciprocal plus one residual correction.
+__device__ inline float fast_div(float a, float b) {
+  const float rb = fast_rcp(b);
+  const float q = a * rb;
+  return __builtin_fmaf(__builtin_fmaf(-b, q, a), rb, q);
+}
 
 // The 11 Jacobian entries and the new x, y, z, phi of one track.
 struct Jac {
   float J02, J03, J04, J05, J12, J13, J14, J15, J42, J43, J45;
 };
 
+// Helix step and Jacobian in FP32 with contraction; the expressions keep the
+// reference's shape (OMP:629-666) so that the only differences from it are
+// FMA rounding and the device sin/cos.
 __device__ inline Jac helix_fast(Track& t, float hz) {
 #pragma clang fp contract(fast)
   float* p = t.p;
@@ -207,39 +265,36 @@ __device__ inline Jac helix_fast(Track& t, float hz) {
   const float ipt = p[3];
   const float pt = fast_rcp(ipt);
   float sinP, cosP, sinT, cosT;
+#ifdef P2Z_FAST_LIBRARY_SINCOS
   sincosf(p[4], &sinP, &cosP);
   sincosf(p[5], &sinT, &cosT);
+#else
+  sincos_accurate(p[4], &sinP, &cosP);
+  sincos_accurate(p[5], &sinT, &cosT);
+#endif
+  const float pxin = cosP * pt;
+  const float pyin = sinP * pt;
   const float icosT = fast_rcp(cosT);
-  const float icosTk = icosT * fast_rcp(k);
+  const float icosTk = fast_div(icosT, k);
   const float alpha = deltaZ * sinT * ipt * icosTk;
   float sina, cosa;
   sincosf(alpha, &sina, &cosa);
-  const float omca = 1.f - cosa;  // exact (Sterbenz) for |alpha| < pi/3
-  const float pxin = cosP * pt, pyin = sinP * pt;
-  p[0] = p[0] + k * (pxin * sina - pyin * omca);
-  p[1] = p[1] + k * (pyin * sina + pxin * omca);
+  p[0] = p[0] + k * (pxin * sina - pyin * (1.f - cosa));
+  p[1] = p[1] + k * (pyin * sina + pxin * (1.f - cosa));
   p[2] = hz;
   p[4] = p[4] + alpha;
 
   float sC, cC;
   sincosf(cosP * sina, &sC, &cC);
-  const float omcC = 1.f - cC;
-  const float sPsC = sinP * sC;
-  const float c2sC = cosP * cosP * sC;
-  const float dzc = deltaZ * cosa;
-  const float iTpt = icosT * pt;
-  const float iT2 = icosT * icosT;
-  const float kpt = k * pt;
-  const float kpt2 = kpt * pt;
   Jac J;
-  J.J02 = cosP * sinT * cosa * (sPsC - 1.f) * icosT;
-  J.J03 = cosP * sinT * dzc * (1.f - sPsC) * iTpt - kpt2 * (cosP * sina - sinP * omcC);
-  J.J04 = kpt * (sinP * sina * (sPsC - 1.f) - cosP * omcC);
-  J.J05 = cosP * dzc * (1.f - sPsC) * iT2;
-  J.J12 = cosa * sinT * (c2sC - sinP) * icosT;
-  J.J13 = sinT * dzc * (c2sC + sinP) * iTpt - kpt2 * (sinP * sina + cosP * omcC);
-  J.J14 = kpt * (cosP * sina - sinP * omcC - cosP * sina * sPsC);
-  J.J15 = dzc * (c2sC + sinP) * iT2;
+  J.J02 = cosP * sinT * (sinP * cosa * sC - cosa) * icosT;
+  J.J03 = cosP * sinT * deltaZ * cosa * (1.f - sinP * sC) * (icosT * pt) - k * (cosP * sina - sinP * (1.f - cC)) * (pt * pt);
+  J.J04 = (k * pt) * (-sinP * sina + sinP * sinP * sina * sC - cosP * (1.f - cC));
+  J.J05 = cosP * deltaZ * cosa * (1.f - sinP * sC) * (icosT * icosT);
+  J.J12 = cosa * sinT * (cosP * cosP * sC - sinP) * icosT;
+  J.J13 = sinT * deltaZ * cosa * (cosP * cosP * sC + sinP) * (icosT * pt) - k * (sinP * sina + cosP * (1.f - cC)) * (pt * pt);
+  J.J14 = (k * pt) * (-sinP * (1.f - cC) - sinP * cosP * sina * sC + cosP * sina);
+  J.J15 = deltaZ * cosa * (cosP * cosP * sC + sinP) * (icosT * icosT);
   J.J42 = -ipt * sinT * icosTk;
   J.J43 = sinT * deltaZ * icosTk;
   J.J45 = ipt * deltaZ * (icosT * icosTk);
@@ -260,20 +315,20 @@ __device__ inline void propagate_cov_fast(float* P, const Jac& J) {
       T0[c] = T1[c] = T4[c] = 0.f;
       continue;
     }
+    // the reference's order (OMP:304-339); with a zero z row the p2 terms
+    // are exact zeros and are dropped
     const float p0 = P[S(0, c)], p1 = P[S(1, c)];
     const float p3 = P[S(3, c)], p4 = P[S(4, c)], p5 = P[S(5, c)];
-    float a = p0 + J.J03 * p3 + J.J04 * p4 + J.J05 * p5;
-    float b = p1 + J.J13 * p3 + J.J14 * p4 + J.J15 * p5;
-    float d = p4 + J.J43 * p3 + J.J45 * p5;
-    if (!kZRow0) {
+    if (kZRow0) {
+      T0[c] = p0 + J.J03 * p3 + J.J04 * p4 + J.J05 * p5;
+      T1[c] = p1 + J.J13 * p3 + J.J14 * p4 + J.J15 * p5;
+      T4[c] = J.J43 * p3 + p4 + J.J45 * p5;
+    } else {
       const float p2 = P[S(2, c)];
-      a += J.J02 * p2;
-      b += J.J12 * p2;
-      d += J.J42 * p2;
+      T0[c] = p0 + J.J02 * p2 + J.J03 * p3 + J.J04 * p4 + J.J05 * p5;
+      T1[c] = p1 + J.J12 * p2 + J.J13 * p3 + J.J14 * p4 + J.J15 * p5;
+      T4[c] = J.J42 * p2 + J.J43 * p3 + p4 + J.J45 * p5;
     }
-    T0[c] = a;
-    T1[c] = b;
-    T4[c] = d;
   }
   float T3[6], T5[6];
 #pragma unroll
@@ -283,9 +338,9 @@ __device__ inline void propagate_cov_fast(float* P, const Jac& J) {
   }
   // (i, 0): Ti0 + Ti2 J02 + Ti3 J03 + Ti4 J04 + Ti5 J05 ; (i, 1) likewise with
   // row 1 of Jt; (i, 3) = Ti3; (i, 4) = Ti2 J42 + Ti3 J43 + Ti4 + Ti5 J45.
-#define P2Z_ROW0(T) (T[0] + (kZRow0 ? 0.f : T[2] * J.J02) + T[3] * J.J03 + T[4] * J.J04 + T[5] * J.J05)
-#define P2Z_ROW1(T) (T[1] + (kZRow0 ? 0.f : T[2] * J.J12) + T[3] * J.J13 + T[4] * J.J14 + T[5] * J.J15)
-#define P2Z_ROW4(T) (T[4] + (kZRow0 ? 0.f : T[2] * J.J42) + T[3] * J.J43 + T[5] * J.J45)
+#define P2Z_ROW0(T) ((kZRow0 ? T[0] : T[0] + T[2] * J.J02) + T[3] * J.J03 + T[4] * J.J04 + T[5] * J.J05)
+#define P2Z_ROW1(T) ((kZRow0 ? T[1] : T[1] + T[2] * J.J12) + T[3] * J.J13 + T[4] * J.J14 + T[5] * J.J15)
+#define P2Z_ROW4(T) ((kZRow0 ? T[3] * J.J43 : T[2] * J.J42 + T[3] * J.J43) + T[4] + T[5] * J.J45)
   P[S(0, 0)] = P2Z_ROW0(T0);
   P[S(1, 0)] = P2Z_ROW0(T1);
   P[S(1, 1)] = P2Z_ROW1(T1);
@@ -336,8 +391,11 @@ __device__ inline void update_fast(Track& t, const Hit& h) {
   const float r00 = P[0] + h.xx;
   const float r01 = P[1] + h.xy;
   const float r11 = P[2] + h.yy;
-  const float s = fast_rcp(r00 * r11 - r01 * r01);
-  const float i00 = s * r11, i01 = -s * r01, i11 = s * r00;
+  // the determinant in FP64 as the reference does (OMP:519-520): exact
+  // products, one rounding; cancellation in it would otherwise dominate
+  const double det = (double)r00 * (double)r11 - (double)r01 * (double)r01;
+  const float s = fast_rcp((float)det);
+  const float i00 = s * r11, i01 = r01 * -s, i11 = s * r00;
   float K0[6], K1[6];
 #pragma unroll
   for (int n = 0; n < L::n; ++n) {
