@@ -130,6 +130,96 @@ int p2z_run(p2z_ctx* ctx, const void* trk, const void* hit, void* out, int64_t n
 int p2z_run_host(p2z_ctx* ctx, const void* trk, const void* hit, void* out, int64_t nevts,
                  int64_t nb, int32_t nlayer, int32_t width, int32_t math);
 
+/* ---- ragged events ----------------------------------------------------------
+ * No counterpart in the reference, whose sizes are uniform compile-time
+ * macros (OMP:18-41).  Events of any track count, tracks of any layer count:
+ * the tracks of an event are sorted by layer count and packed into width-W
+ * MPTRK blocks (blocks never mix events; the last one is padded with lanes of
+ * layer count 0, copied through unchanged).  A block with n layer slots owns n
+ * consecutive MPHIT blocks; slot L holds layer L of the lanes that have it.
+ * lane_nlayer holds each track's layer count in MP1I layout (block b lane l at
+ * b*W + l); a lane stops after its own count. */
+typedef struct {
+  int64_t trk_block; /* MPTRK block (input and output index)              */
+  int64_t hit_block; /* first of the block's MPHIT blocks                 */
+  int32_t nlayer;    /* layer slots of the block (max of its lanes)       */
+  int32_t event;     /* owning event                                      */
+} p2z_block_desc;
+
+#define P2Z_RAGGED_PERSISTENT 1u /* one persistent launch draining a work queue
+                                    (descriptor order = processing order);
+                                    otherwise one wave per 64 track slots     */
+#define P2Z_RAGGED_GRAPH 2u      /* launch through a captured hipGraph, cached
+                                    per argument set (persistent form only)   */
+
+typedef struct {
+  int32_t tracks_per_wg; /* 0: the context's setting                          */
+  int32_t prefetch;      /* hit layers loaded ahead: 1 or 2; 0 = 1             */
+  int32_t waves_per_cu;  /* persistent grid = CUs * waves_per_cu waves; 0 = 8  */
+  int32_t reserved;
+} p2z_ragged_opts;
+
+/* Runs every block of the descriptor table (device memory) through its layers.
+ * trk, hit, out, lane_nlayer, blocks: device pointers.  Asynchronous. */
+int p2z_run_ragged(p2z_ctx* ctx, const void* trk, const void* hit, void* out,
+                   const int32_t* lane_nlayer, const p2z_block_desc* blocks, int64_t nblocks,
+                   int32_t width, int32_t math, uint32_t flags, const p2z_ragged_opts* opts);
+
+/* ---- synthetic workloads (host) ----------------------------------------------
+ * Helix-consistent events: each track's true helix (the path's own helix
+ * model, in double) is intersected with its layer planes and the hits are
+ * the true points smeared by their covariance; the starting state is the
+ * truth smeared by its covariance.  Deterministic for a seed, independent of
+ * thread count (one random stream per event).
+ *   P2Z_WORKLOAD_PRECISION: nevents x ntracks x nlayer (uniform), 1/pT
+ *     log-uniform [0.01,2] GeV^-1, theta U[0.1,3.04], phi U[-pi,pi], hit
+ *     covariance diagonals log-uniform [1e-8,1e-3] cm^2.
+ *   P2Z_WORKLOAD_RAGGED: tracks per event log-normal (median 2000, sigma 1)
+ *     clipped to [50,20000], layers per track U{8..26}, layer spacing
+ *     U[5,25] cm, |cos theta| >= 0.35, hit covariance diagonals
+ *     log-uniform [1e-6,1e-3] cm^2. */
+#define P2Z_WORKLOAD_PRECISION 2
+#define P2Z_WORKLOAD_RAGGED 3
+
+typedef struct {
+  int32_t kind;    /* P2Z_WORKLOAD_*                                 */
+  int32_t width;   /* AoSoA width                                    */
+  int64_t nevents;
+  uint64_t seed;
+  int32_t ntracks; /* PRECISION: tracks per event (RAGGED: ignored)   */
+  int32_t nlayer;  /* PRECISION: layers per track (RAGGED: ignored)   */
+  int32_t nthreads;/* host threads for generation; 0 = all            */
+  int32_t reserved;
+} p2z_workload_spec;
+
+typedef struct {
+  int64_t nevents, ntracks, nblocks, nhit_blocks, track_layers;
+  int32_t width, kind;
+} p2z_workload_info;
+
+typedef struct p2z_workload p2z_workload;
+int p2z_workload_create(const p2z_workload_spec* spec, p2z_workload** out);
+int p2z_workload_destroy(p2z_workload* w);
+int p2z_workload_get_info(const p2z_workload* w, p2z_workload_info* info);
+/* Host arrays owned by the workload (valid until destroy):
+ *   tracks  : nblocks MPTRK blocks           hits   : nhit_blocks MPHIT blocks
+ *   lane_nlayer : nblocks*W int32            blocks : nblocks descriptors in
+ *   event order; blocks_lpt: the same descriptors sorted by layer count,
+ *   longest first (the persistent kernel's processing order);
+ *   event_blocks: nevents+1 offsets of each event's blocks in event order;
+ *   event_tracks: nevents track counts. */
+const void* p2z_workload_tracks(const p2z_workload* w);
+const void* p2z_workload_hits(const p2z_workload* w);
+const int32_t* p2z_workload_lane_nlayer(const p2z_workload* w);
+const p2z_block_desc* p2z_workload_blocks(const p2z_workload* w);
+const p2z_block_desc* p2z_workload_blocks_lpt(const p2z_workload* w);
+const int64_t* p2z_workload_event_blocks(const p2z_workload* w);
+const int32_t* p2z_workload_event_tracks(const p2z_workload* w);
+/* Binary fixture: a JSON header line (sizes, spec, checksums) followed by the
+ * arrays above; load verifies the checksums. */
+int p2z_workload_save(const p2z_workload* w, const char* path);
+int p2z_workload_load(const char* path, p2z_workload** out);
+
 /* ---- report (device) ------------------------------------------------------
  * The statistics of OMP:758-876 computed on the GPU from device-resident
  * output: double accumulation in a fixed order (deterministic), same

@@ -327,6 +327,31 @@ void oracle_kalman_update_v2(float* trkErr, float* inPar, const float* hitErr, c
   }
 }
 
+void oracle_propagate_to_z_fp64(const float* inErr, const float* inPar, const int32_t* inChg,
+                                const float* msP, float* outErr, float* outPar, int W) {
+  for (int it = 0; it < W; ++it) {
+    double P[21], par[6];
+    for (int e = 0; e < 21; ++e) P[e] = inErr[e * W + it];
+    for (int e = 0; e < 6; ++e) par[e] = inPar[e * W + it];
+    lane_propagate64(P, par, inChg[it], msP[2 * W + it]);
+    for (int e = 0; e < 21; ++e) outErr[e * W + it] = (float)P[e];
+    for (int e = 0; e < 6; ++e) outPar[e * W + it] = (float)par[e];
+  }
+}
+
+void oracle_kalman_update_v2_fp64(float* trkErr, float* inPar, const float* hitErr, const float* msP,
+                                  int W) {
+  for (int it = 0; it < W; ++it) {
+    double P[21], par[6];
+    for (int e = 0; e < 21; ++e) P[e] = trkErr[e * W + it];
+    for (int e = 0; e < 6; ++e) par[e] = inPar[e * W + it];
+    const double hc[3] = {hitErr[0 * W + it], hitErr[1 * W + it], hitErr[2 * W + it]};
+    lane_update64(P, par, hc, msP[0 * W + it], msP[1 * W + it]);
+    for (int e = 0; e < 21; ++e) trkErr[e * W + it] = (float)P[e];
+    for (int e = 0; e < 6; ++e) inPar[e * W + it] = (float)par[e];
+  }
+}
+
 /* One block through nlayer layers (body of OMP:736-746). */
 static void run_block(const float* tb, const float* hb0, float* ob, int W, int nlayer) {
   memcpy(ob, tb, sizeof(float) * ORACLE_TRK_WORDS * W);

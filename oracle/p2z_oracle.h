@@ -64,6 +64,13 @@ void oracle_run_events_only(const float* trk, const float* hit, float* out,
                             int64_t nevts, int64_t nb, int W, int nlayer, int nthreads);
 
 /* ---- full-FP64 oracle ---------------------------------------------------- */
+/* propagateToZ / KalmanUpdate_v2 on one block computed entirely in double,
+ * results rounded to float (same block layout and aliasing rules). */
+void oracle_propagate_to_z_fp64(const float* inErr, const float* inPar, const int32_t* inChg,
+                                const float* msP, float* outErr, float* outPar, int W);
+void oracle_kalman_update_v2_fp64(float* trkErr, float* inPar, const float* hitErr,
+                                  const float* msP, int W);
+
 /* Same algorithm with every quantity in double (libm sin/cos in double); the
  * state is kept in double between layers.  Inputs are the FP32 blocks; the
  * outputs are 28 doubles per track in track-major order:

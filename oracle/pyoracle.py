@@ -50,6 +50,8 @@ _SIGS = {
     "oracle_prepare_hits": ([C.POINTER(OAHIT), C.c_int, vp, i64, i64, C.c_int, C.c_int, dbl], C.c_int),
     "oracle_propagate_to_z": ([vp, vp, vp, vp, vp, vp, C.c_int], None),
     "oracle_kalman_update_v2": ([vp, vp, vp, vp, C.c_int], None),
+    "oracle_propagate_to_z_fp64": ([vp, vp, vp, vp, vp, vp, C.c_int], None),
+    "oracle_kalman_update_v2_fp64": ([vp, vp, vp, vp, C.c_int], None),
     "oracle_run": ([vp, vp, vp, i64, i64, C.c_int, C.c_int, C.c_int], None),
     "oracle_run_events_only": ([vp, vp, vp, i64, i64, C.c_int, C.c_int, C.c_int], None),
     "oracle_run_fp64": ([vp, vp, vp, i64, i64, C.c_int, C.c_int, C.c_int], None),
@@ -180,6 +182,23 @@ def update_block(trk_blk, hit_blk, width) -> np.ndarray:
     W = width
     out = trk_blk.copy()
     lib().oracle_kalman_update_v2(_p(out) + 6 * W * 4, _p(out), _p(hit_blk) + 3 * W * 4, _p(hit_blk), W)
+    return out
+
+
+def propagate_block_fp64(trk_blk, hit_blk, width) -> np.ndarray:
+    """propagateToZ on one block in double, rounded to float."""
+    W = width
+    out = trk_blk.copy()
+    lib().oracle_propagate_to_z_fp64(_p(trk_blk) + 6 * W * 4, _p(trk_blk), _p(trk_blk) + 27 * W * 4, _p(hit_blk),
+                                     _p(out) + 6 * W * 4, _p(out), W)
+    return out
+
+
+def update_block_fp64(trk_blk, hit_blk, width) -> np.ndarray:
+    """KalmanUpdate_v2 on one block in double, rounded to float."""
+    W = width
+    out = trk_blk.copy()
+    lib().oracle_kalman_update_v2_fp64(_p(out) + 6 * W * 4, _p(out), _p(hit_blk) + 3 * W * 4, _p(hit_blk), W)
     return out
 
 

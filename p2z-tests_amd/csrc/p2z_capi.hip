@@ -5,7 +5,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "p2z.h"
@@ -27,6 +29,12 @@ struct p2z_ctx {
   double* sum_part = nullptr;
   size_t sum_part_len = 0;
   double* sum_res = nullptr;
+  // ragged path: work-queue counter, compute units, captured graphs
+  unsigned* counter = nullptr;
+  int cus = 0;
+  using GraphKey = std::tuple<const void*, const void*, void*, const void*, const void*, int64_t, int, int, int,
+                              int, int64_t>;
+  std::map<GraphKey, hipGraphExec_t> graphs;
 };
 
 namespace {
@@ -112,6 +120,10 @@ int p2z_create(p2z_ctx** out, int device, const p2z_opts* opts) {
   if (rc == P2Z_OK) rc = p2z::hip_status(hipEventCreate(&c->t1), "hipEventCreate");
   if (rc == P2Z_OK) rc = apply_opts(c, opts);
   if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc((void**)&c->sum_res, 32 * sizeof(double)), "hipMalloc");
+  if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc((void**)&c->counter, 256), "hipMalloc");
+  if (rc == P2Z_OK)
+    rc = p2z::hip_status(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device),
+                         "hipDeviceGetAttribute");
   if (rc != P2Z_OK) {
     p2z_destroy(c);
     return rc;
@@ -131,6 +143,8 @@ int p2z_destroy(p2z_ctx* c) {
   for (auto s : c->pipe) (void)hipStreamDestroy(s);
   if (c->sum_part) (void)hipFree(c->sum_part);
   if (c->sum_res) (void)hipFree(c->sum_res);
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (c->counter) (void)hipFree(c->counter);
   if (c->t0) (void)hipEventDestroy(c->t0);
   if (c->t1) (void)hipEventDestroy(c->t1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -319,6 +333,67 @@ int p2z_run_host(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_
                                  hipMemcpyDeviceToHost, s));
   }
   for (auto s : c->pipe) P2Z_HIP_CHECK(hipStreamSynchronize(s));
+  return P2Z_OK;
+}
+
+int p2z_run_ragged(p2z_ctx* c, const void* trk, const void* hit, void* out, const int32_t* lane_nlayer,
+                   const p2z_block_desc* blocks, int64_t nblocks, int32_t width, int32_t math, uint32_t flags,
+                   const p2z_ragged_opts* opts) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  if (!trk || !hit || !out || !lane_nlayer || !blocks) return p2z::fail(P2Z_EINVAL, "null buffer");
+  if (!p2z::valid_width(width)) return p2z::fail(P2Z_EINVAL, "width must be 16/32/64/128 (got %d)", width);
+  if (!p2z::valid_math(math)) return p2z::fail(P2Z_EINVAL, "unknown math mode %d", math);
+  if (nblocks < 0 || nblocks * (int64_t)width > ((int64_t)1 << 31) - 64)
+    return p2z::fail(P2Z_EINVAL, "bad block count %lld", (long long)nblocks);
+  if (flags & ~(P2Z_RAGGED_PERSISTENT | P2Z_RAGGED_GRAPH)) return p2z::fail(P2Z_EINVAL, "unknown flags 0x%x", flags);
+  if ((flags & P2Z_RAGGED_GRAPH) && !(flags & P2Z_RAGGED_PERSISTENT))
+    return p2z::fail(P2Z_EINVAL, "P2Z_RAGGED_GRAPH needs P2Z_RAGGED_PERSISTENT");
+  int tpw = c->cfg.tracks_per_wg, prefetch = 1, wpc = 8;
+  if (opts) {
+    if (opts->tracks_per_wg) {
+      if (opts->tracks_per_wg < 64 || opts->tracks_per_wg > 256 || opts->tracks_per_wg % 64)
+        return p2z::fail(P2Z_EINVAL, "tracks_per_wg must be 64, 128, 192 or 256");
+      tpw = opts->tracks_per_wg;
+    }
+    if (opts->prefetch) {
+      if (opts->prefetch < 1 || opts->prefetch > 2) return p2z::fail(P2Z_EINVAL, "prefetch must be 1 or 2");
+      prefetch = opts->prefetch;
+    }
+    if (opts->waves_per_cu) {
+      if (opts->waves_per_cu < 1 || opts->waves_per_cu > 32) return p2z::fail(P2Z_EINVAL, "waves_per_cu in [1,32]");
+      wpc = opts->waves_per_cu;
+    }
+  }
+  P2Z_TRY(use_device(c));
+  if (nblocks == 0) return P2Z_OK;
+  const bool persistent = flags & P2Z_RAGGED_PERSISTENT;
+  const int64_t grid = ((int64_t)c->cus * wpc * 64 + tpw - 1) / tpw;
+  const auto* bd = reinterpret_cast<const p2z::BlockDesc*>(blocks);
+  auto launch = [&](hipStream_t s) {
+    return p2z::launch_ragged((const float*)trk, (const float*)hit, (float*)out, lane_nlayer, bd, nblocks, width,
+                              math, persistent, prefetch, tpw, grid, c->counter, s);
+  };
+  if (!(flags & P2Z_RAGGED_GRAPH)) {
+    P2Z_HIP_CHECK(launch(c->stream));
+    return P2Z_OK;
+  }
+  // counter reset + persistent kernel, captured once per argument set
+  const p2z_ctx::GraphKey key{trk, hit, out, lane_nlayer, blocks, nblocks, width, math, prefetch, tpw, grid};
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    P2Z_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t le = launch(c->stream);
+    const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+    P2Z_HIP_CHECK(le);
+    P2Z_HIP_CHECK(ce);
+    const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    P2Z_HIP_CHECK(ie);
+    it = c->graphs.emplace(key, ge).first;
+  }
+  P2Z_HIP_CHECK(hipGraphLaunch(it->second, c->stream));
   return P2Z_OK;
 }
 

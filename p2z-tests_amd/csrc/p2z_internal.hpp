@@ -31,9 +31,25 @@ hipError_t launch_update(float* trkErr, float* inPar, const float* hitErr, const
                          int64_t nblocks, int64_t ts, int64_t hs, int width, int math,
                          const LaunchCfg& cfg, hipStream_t s);
 
+// Block descriptor of the ragged format (same layout as p2z_block_desc).
+struct BlockDesc {
+  int64_t trk_block;
+  int64_t hit_block;
+  int32_t nlayer;
+  int32_t event;
+};
+static_assert(sizeof(BlockDesc) == 24, "descriptor layout");
+
+// Ragged pass.  persistent: grid workgroups drain the descriptor table through
+// the device counter (reset first on the same stream); otherwise one wave per
+// 64 track slots.
+hipError_t launch_ragged(const float* trk, const float* hit, float* out, const int32_t* lane_nlay,
+                         const BlockDesc* blocks, int64_t nblocks, int width, int math, bool persistent,
+                         int prefetch, int tpw, int64_t grid, unsigned* counter, hipStream_t s);
+
 // Report statistics (OMP:758-851) on device.  part: scratch of
-// summary_scratch_doubles(nevts, ntrks) doubles.  res: 16 doubles on device
-// (9 sums / 6 sums of squares, counts as doubles), written by the final pass.
+// summary_scratch_doubles(nevts, ntrks) doubles.  res: 18 doubles on device
+// (9 means, 3 counts, 6 RMS values), written by the final passes.
 size_t summary_scratch_doubles(int64_t nevts, int64_t ntrks);
 hipError_t launch_summary(const float* out, int64_t nevts, int64_t nb, int width, int64_t ntrks,
                           double hx, double hy, double hz, double* part, double* res,

@@ -9,6 +9,8 @@
 #include "p2z_internal.hpp"
 #include "p2z_lane.hpp"
 
+#include <algorithm>
+
 namespace p2z {
 namespace {
 
@@ -74,6 +76,92 @@ __global__ __launch_bounds__(256) void k_fused(const float* __restrict__ trk,
 }
 
 // ---------------------------------------------------------------------------
+// ragged events: per-block descriptors, per-lane layer counts
+// ---------------------------------------------------------------------------
+
+// One track of a descriptor's block through its own layer count, with the
+// hits of the next kPrefetch layers in flight while a layer is computed.
+template <int W, int kMath, int kPrefetch>
+__device__ inline void ragged_track(const float* __restrict__ trk, const float* __restrict__ hit,
+                                    float* __restrict__ out, const int32_t* __restrict__ lane_nlay,
+                                    const BlockDesc& d, int l) {
+  Track t;
+  load_track<W>(t, trk + d.trk_block * (kTrkWords * W) + l);
+  int n = lane_nlay[d.trk_block * W + l];
+  n = n < d.nlayer ? n : d.nlayer;
+  const float* hb = hit + d.hit_block * (kHitWords * W) + l;
+  if (n > 0) {
+    Hit h0 = load_hit<W>(hb), h1 = h0, h2 = h0;
+    if (n > 1) h1 = load_hit<W>(hb + kHitWords * W);
+    if (kPrefetch >= 2 && n > 2) h2 = load_hit<W>(hb + 2 * kHitWords * W);
+    layer_step<kMath, true>(t, h0);
+    for (int lay = 1; lay < n; ++lay) {
+      h0 = h1;
+      if (kPrefetch >= 2) {
+        h1 = h2;
+        if (lay + 2 < n) h2 = load_hit<W>(hb + (int64_t)(lay + 2) * (kHitWords * W));
+      } else if (lay + 1 < n) {
+        h1 = load_hit<W>(hb + (int64_t)(lay + 1) * (kHitWords * W));
+      }
+      layer_step<kMath, false>(t, h0);
+    }
+  }
+  store_track<W>(t, out + d.trk_block * (kTrkWords * W) + l);
+}
+
+// Work item = 64 consecutive track slots of the descriptor table (one wave):
+// slot s is lane s % W of descriptor s / W.
+template <int W, int kMath, int kPrefetch>
+__device__ inline void ragged_item(const float* trk, const float* hit, float* out, const int32_t* lane_nlay,
+                                   const BlockDesc* __restrict__ blocks, int64_t nblocks, int64_t item,
+                                   int lane) {
+  const int64_t slot = item * 64 + lane;
+  const int64_t di = slot / W;
+  if (di >= nblocks) return;
+  const BlockDesc d = blocks[di];
+  ragged_track<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, d, (int)(slot % W));
+}
+
+// Persistent form: each wave takes the next item from a device counter until
+// the items run out (every wave reaches that exit).  The counter starts at 0
+// (k_reset_counter runs first on the same stream).
+template <int W, int kMath, int kPrefetch>
+__global__ __launch_bounds__(256) void k_ragged_persistent(const float* __restrict__ trk,
+                                                           const float* __restrict__ hit,
+                                                           float* __restrict__ out,
+                                                           const int32_t* __restrict__ lane_nlay,
+                                                           const BlockDesc* __restrict__ blocks,
+                                                           int64_t nblocks, unsigned* __restrict__ counter) {
+  const int64_t nitems = (nblocks * W + 63) / 64;
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    unsigned mine = 0;
+    if (lane == 0) mine = atomicAdd(counter, 1u);
+    const int64_t item = (int64_t)(unsigned)__shfl((int)mine, 0);
+    if (item >= nitems) break;
+    ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, lane);
+  }
+}
+
+// One-shot form: one wave per item.
+template <int W, int kMath>
+__global__ __launch_bounds__(256) void k_ragged_oneshot(const float* __restrict__ trk,
+                                                        const float* __restrict__ hit,
+                                                        float* __restrict__ out,
+                                                        const int32_t* __restrict__ lane_nlay,
+                                                        const BlockDesc* __restrict__ blocks,
+                                                        int64_t nblocks) {
+  const int64_t nitems = (nblocks * W + 63) / 64;
+  const int64_t item = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  if (item >= nitems) return;
+  ragged_item<W, kMath, 1>(trk, hit, out, lane_nlay, blocks, nblocks, item, threadIdx.x & 63);
+}
+
+__global__ void k_reset_counter(unsigned* counter) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
+}
+
+// ---------------------------------------------------------------------------
 // per-function kernels (reference propagateToZ / KalmanUpdate_v2 on blocks)
 // ---------------------------------------------------------------------------
 template <int W, int kMath>
@@ -96,7 +184,7 @@ __global__ __launch_bounds__(256) void k_propagate(const float* inErr, const flo
   if constexpr (kMath == kRef)
     propagate_ref(t, hz);
   else
-    propagate_fast<false>(t, hz);
+    propagate_fast<false, kMath == kFp64Gain>(t, hz);
 #pragma unroll
   for (int e = 0; e < 21; ++e) outErr[o + e * W] = t.P[e];
 #pragma unroll
@@ -331,6 +419,34 @@ hipError_t launch_update(float* trkErr, float* inPar, const float* hitErr, const
       constexpr int W = decltype(wc)::value, M = decltype(mc)::value;
       hipLaunchKernelGGL((k_update<W, M>), dim3((unsigned)grid_for(ntracks, tpw)), dim3(tpw), 0, s,
                          trkErr, inPar, hitErr, msP, ntracks, ts, hs);
+      return hipGetLastError();
+    });
+  });
+}
+
+hipError_t launch_ragged(const float* trk, const float* hit, float* out, const int32_t* lane_nlay,
+                         const BlockDesc* blocks, int64_t nblocks, int width, int math, bool persistent,
+                         int prefetch, int tpw, int64_t grid, unsigned* counter, hipStream_t s) {
+  const int64_t nitems = (nblocks * width + 63) / 64;
+  if (nitems == 0) return hipSuccess;
+  return dispatch_width(width, [&](auto wc) {
+    return dispatch_math(math, [&](auto mc) {
+      constexpr int W = decltype(wc)::value, M = decltype(mc)::value;
+      if (!persistent) {
+        hipLaunchKernelGGL((k_ragged_oneshot<W, M>), dim3((unsigned)grid_for(nitems * 64, tpw)), dim3(tpw), 0, s,
+                           trk, hit, out, lane_nlay, blocks, nblocks);
+        return hipGetLastError();
+      }
+      hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, counter);
+      const int64_t items_per_wg = tpw / 64;
+      int64_t g = grid > 0 ? grid : 1;
+      g = std::min<int64_t>(g, (nitems + items_per_wg - 1) / items_per_wg);
+      if (prefetch >= 2)
+        hipLaunchKernelGGL((k_ragged_persistent<W, M, 2>), dim3((unsigned)g), dim3(tpw), 0, s, trk, hit, out,
+                           lane_nlay, blocks, nblocks, counter);
+      else
+        hipLaunchKernelGGL((k_ragged_persistent<W, M, 1>), dim3((unsigned)g), dim3(tpw), 0, s, trk, hit, out,
+                           lane_nlay, blocks, nblocks, counter);
       return hipGetLastError();
     });
   });

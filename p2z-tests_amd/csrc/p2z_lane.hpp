@@ -257,6 +257,11 @@ struct Jac {
 // Helix step and Jacobian in FP32 with contraction; the expressions keep the
 // reference's shape (OMP:629-666) so that the only differences from it are
 // FMA rounding and the device sin/cos.
+// kAccurateTrig: the track angles' sin/cos through sincos_accurate (the
+// FP64-gain mode); otherwise the library's sincosf, which measures as accurate
+// as the reference at the workload level (error against an FP64 evaluation no
+// larger than the reference's own) at lower cost.
+template <bool kAccurateTrig>
 __device__ inline Jac helix_fast(Track& t, float hz) {
 #pragma clang fp contract(fast)
   float* p = t.p;
@@ -265,27 +270,33 @@ __device__ inline Jac helix_fast(Track& t, float hz) {
   const float ipt = p[3];
   const float pt = fast_rcp(ipt);
   float sinP, cosP, sinT, cosT;
-#ifdef P2Z_FAST_LIBRARY_SINCOS
-  sincosf(p[4], &sinP, &cosP);
-  sincosf(p[5], &sinT, &cosT);
-#else
-  sincos_accurate(p[4], &sinP, &cosP);
-  sincos_accurate(p[5], &sinT, &cosT);
-#endif
+  if constexpr (kAccurateTrig) {
+    sincos_accurate(p[4], &sinP, &cosP);
+    sincos_accurate(p[5], &sinT, &cosT);
+  } else {
+    sincosf(p[4], &sinP, &cosP);
+    sincosf(p[5], &sinT, &cosT);
+  }
   const float pxin = cosP * pt;
   const float pyin = sinP * pt;
   const float icosT = fast_rcp(cosT);
   const float icosTk = fast_div(icosT, k);
   const float alpha = deltaZ * sinT * ipt * icosTk;
   float sina, cosa;
-  sincosf(alpha, &sina, &cosa);
+  if constexpr (kAccurateTrig)
+    sincos_accurate(alpha, &sina, &cosa);
+  else
+    sincosf(alpha, &sina, &cosa);
   p[0] = p[0] + k * (pxin * sina - pyin * (1.f - cosa));
   p[1] = p[1] + k * (pyin * sina + pxin * (1.f - cosa));
   p[2] = hz;
   p[4] = p[4] + alpha;
 
   float sC, cC;
-  sincosf(cosP * sina, &sC, &cC);
+  if constexpr (kAccurateTrig)
+    sincos_accurate(cosP * sina, &sC, &cC);
+  else
+    sincosf(cosP * sina, &sC, &cC);
   Jac J;
   J.J02 = cosP * sinT * (sinP * cosa * sC - cosa) * icosT;
   J.J03 = cosP * sinT * deltaZ * cosa * (1.f - sinP * sC) * (icosT * pt) - k * (cosP * sina - sinP * (1.f - cC)) * (pt * pt);
@@ -367,9 +378,9 @@ __device__ inline void propagate_cov_fast(float* P, const Jac& J) {
   P[S(5, 2)] = 0.f;
 }
 
-template <bool kZRow0>
+template <bool kZRow0, bool kAccurateTrig = false>
 __device__ inline void propagate_fast(Track& t, float hz) {
-  const Jac J = helix_fast(t, hz);
+  const Jac J = helix_fast<kAccurateTrig>(t, hz);
   propagate_cov_fast<kZRow0>(t.P, J);
 }
 
@@ -499,7 +510,7 @@ __device__ inline void layer_step(Track& t, const Hit& h) {
     propagate_fast<!kFirst>(t, h.z);
     update_fast<true>(t, h);
   } else {
-    propagate_fast<!kFirst>(t, h.z);
+    propagate_fast<!kFirst, true>(t, h.z);
     update_fp64gain<true>(t, h);
   }
 }

@@ -19,7 +19,8 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libp2z.so")
+# P2Z_LIBRARY selects another build of the library (same ABI), e.g. a variant.
+LIB_PATH = os.environ.get("P2Z_LIBRARY") or os.path.join(HERE, "lib", "libp2z.so")
 MAIN_PATH = os.path.join(HERE, "bin", "p2z_main")
 
 TRK_WORDS = 28
@@ -64,6 +65,31 @@ class Summary(C.Structure):
         return d
 
 
+class BlockDesc(C.Structure):
+    _fields_ = [("trk_block", C.c_int64), ("hit_block", C.c_int64), ("nlayer", C.c_int32), ("event", C.c_int32)]
+
+
+BLOCK_DESC_DTYPE = np.dtype([("trk_block", "<i8"), ("hit_block", "<i8"), ("nlayer", "<i4"), ("event", "<i4")])
+
+
+class RaggedOpts(C.Structure):
+    _fields_ = [("tracks_per_wg", C.c_int32), ("prefetch", C.c_int32), ("waves_per_cu", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class WorkloadSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("width", C.c_int32), ("nevents", C.c_int64), ("seed", C.c_uint64),
+                ("ntracks", C.c_int32), ("nlayer", C.c_int32), ("nthreads", C.c_int32), ("reserved", C.c_int32)]
+
+
+class WorkloadInfo(C.Structure):
+    _fields_ = [("nevents", C.c_int64), ("ntracks", C.c_int64), ("nblocks", C.c_int64), ("nhit_blocks", C.c_int64),
+                ("track_layers", C.c_int64), ("width", C.c_int32), ("kind", C.c_int32)]
+
+
+RAGGED_PERSISTENT, RAGGED_GRAPH = 1, 2
+WORKLOAD = {"precision": 2, "ragged": 3}
+
 _lib = None
 vp, i32, i64, u32, dbl, sz = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_double, C.c_size_t
 
@@ -93,6 +119,19 @@ _SIGS = {
     "p2z_kalman_update": ([vp, vp, vp, vp, vp, i64, i64, i64, i32, i32], i32),
     "p2z_run": ([vp, vp, vp, vp, i64, i64, i32, i32, i32], i32),
     "p2z_run_host": ([vp, vp, vp, vp, i64, i64, i32, i32, i32], i32),
+    "p2z_run_ragged": ([vp, vp, vp, vp, vp, vp, i64, i32, i32, u32, C.POINTER(RaggedOpts)], i32),
+    "p2z_workload_create": ([C.POINTER(WorkloadSpec), C.POINTER(vp)], i32),
+    "p2z_workload_destroy": ([vp], i32),
+    "p2z_workload_get_info": ([vp, C.POINTER(WorkloadInfo)], i32),
+    "p2z_workload_tracks": ([vp], vp),
+    "p2z_workload_hits": ([vp], vp),
+    "p2z_workload_lane_nlayer": ([vp], vp),
+    "p2z_workload_blocks": ([vp], vp),
+    "p2z_workload_blocks_lpt": ([vp], vp),
+    "p2z_workload_event_blocks": ([vp], vp),
+    "p2z_workload_event_tracks": ([vp], vp),
+    "p2z_workload_save": ([vp, C.c_char_p], i32),
+    "p2z_workload_load": ([C.c_char_p, C.POINTER(vp)], i32),
     "p2z_summary_device": ([vp, vp, i64, i64, i32, i64, C.POINTER(C.c_float), C.POINTER(Summary)], i32),
     "p2z_summary_host": ([vp, i64, i64, i32, i64, C.POINTER(C.c_float), C.POINTER(Summary)], i32),
     "p2z_format_summary": ([C.POINTER(Summary), C.c_char_p, i32], i32),
@@ -238,6 +277,67 @@ def device_count() -> int:
 
 
 # ---------------------------------------------------------------------------
+# synthetic workloads (ragged-event format)
+# ---------------------------------------------------------------------------
+
+def _view(ptr: int, dtype, count: int) -> np.ndarray:
+    if count == 0:
+        return np.empty(0, dtype)
+    buf = (C.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+    return np.frombuffer(buf, dtype=dtype, count=count)
+
+
+class Workload:
+    """A generated (or loaded) workload in the ragged format.  The numpy
+    arrays are views of library memory, valid while the object lives."""
+
+    def __init__(self, kind: str = "ragged", nevents: int = 2000, width: int = 32, seed: int = 1,
+                 ntracks: int = 9600, nlayer: int = 20, nthreads: int = 0, _handle=None):
+        if _handle is None:
+            spec = WorkloadSpec(WORKLOAD[kind], width, nevents, seed, ntracks, nlayer, nthreads, 0)
+            h = vp()
+            check(lib().p2z_workload_create(C.byref(spec), C.byref(h)), "p2z_workload_create")
+            self.handle = h
+        else:
+            self.handle = _handle
+        info = WorkloadInfo()
+        check(lib().p2z_workload_get_info(self.handle, C.byref(info)), "p2z_workload_get_info")
+        self.info = info
+        W = info.width
+        L = lib()
+        self.width = W
+        self.nevents, self.ntracks, self.nblocks = info.nevents, info.ntracks, info.nblocks
+        self.nhit_blocks, self.track_layers = info.nhit_blocks, info.track_layers
+        self.tracks = _view(L.p2z_workload_tracks(self.handle), np.float32, info.nblocks * TRK_WORDS * W)
+        self.hits = _view(L.p2z_workload_hits(self.handle), np.float32, info.nhit_blocks * HIT_WORDS * W)
+        self.lane_nlayer = _view(L.p2z_workload_lane_nlayer(self.handle), np.int32, info.nblocks * W)
+        self.blocks = _view(L.p2z_workload_blocks(self.handle), BLOCK_DESC_DTYPE, info.nblocks)
+        self.blocks_lpt = _view(L.p2z_workload_blocks_lpt(self.handle), BLOCK_DESC_DTYPE, info.nblocks)
+        self.event_blocks = _view(L.p2z_workload_event_blocks(self.handle), np.int64, info.nevents + 1)
+        self.event_tracks = _view(L.p2z_workload_event_tracks(self.handle), np.int32, info.nevents)
+
+    @classmethod
+    def load(cls, path: str) -> "Workload":
+        h = vp()
+        check(lib().p2z_workload_load(path.encode(), C.byref(h)), "p2z_workload_load")
+        return cls(_handle=h)
+
+    def save(self, path: str) -> None:
+        check(lib().p2z_workload_save(self.handle, path.encode()), "p2z_workload_save")
+
+    def close(self) -> None:
+        if self.handle:
+            lib().p2z_workload_destroy(self.handle)
+            self.handle = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
 # device side
 # ---------------------------------------------------------------------------
 
@@ -349,6 +449,17 @@ class Context:
         f4 = 4
         check(lib().p2z_kalman_update(self.handle, trk.ptr + 6 * W * f4, trk.ptr, hit.ptr + 3 * W * f4, hit.ptr,
                                       nblocks, TRK_WORDS * W, hs, W, MATH.get(math, math)), "p2z_kalman_update")
+
+    def run_ragged(self, trk, hit, out, lane_nlayer, blocks, nblocks: int, width: int, math="fast",
+                   persistent: bool = True, graph: bool = False, tracks_per_wg: int = 0, prefetch: int = 0,
+                   waves_per_cu: int = 0) -> None:
+        """All blocks of a descriptor table (device buffer of BlockDesc) through
+        their layers: one persistent work-queue launch (optionally replayed from
+        a captured hipGraph), or one wave per 64 track slots."""
+        flags = (RAGGED_PERSISTENT if persistent else 0) | (RAGGED_GRAPH if graph else 0)
+        o = RaggedOpts(tracks_per_wg, prefetch, waves_per_cu, 0)
+        check(lib().p2z_run_ragged(self.handle, _ptr(trk), _ptr(hit), _ptr(out), _ptr(lane_nlayer), _ptr(blocks),
+                                   nblocks, width, MATH.get(math, math), flags, C.byref(o)), "p2z_run_ragged")
 
     def summary_device(self, out: DeviceBuffer, nevts: int, nb: int, width: int, ntrks: int, lasthit) -> Summary:
         s = Summary()

@@ -48,8 +48,10 @@ def compare(a: np.ndarray, b: np.ndarray) -> dict:
         nonfinite_mismatch=int((fin_a != fin_b).sum()),
         charge_mismatch=int((a[:, 27] != b[:, 27]).sum()),
         par_max=float(par.max()) if par.size else 0.0,
+        par_p99=float(np.quantile(par.max(axis=1), 0.99)) if par.size else 0.0,
         par_median=float(np.median(par)) if par.size else 0.0,
         cov_max=float(cov.max()) if cov.size else 0.0,
+        cov_p99=float(np.quantile(cov.max(axis=1), 0.99)) if cov.size else 0.0,
         cov_median=float(np.median(cov)) if cov.size else 0.0,
         par_max_by_elem=[float(v) for v in par.max(axis=0)] if par.size else [],
     )

@@ -1,0 +1,167 @@
+"""GPU tests of the ragged-event path (p2z_run_ragged): persistent work-queue
+launch, hipGraph replay, one-shot launch, all widths, against the oracle's
+ragged driver and against the plain fused path."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import p2z
+import pyoracle as O
+from parity import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg: str) -> None:
+    print(msg, flush=True)
+
+
+def by_track(w: "p2z.Workload", out: np.ndarray) -> np.ndarray:
+    """(ntracks, 28) float64 in event / sorted-track order, padding lanes dropped."""
+    W = w.width
+    t = out.reshape(w.nblocks, 28, W).transpose(0, 2, 1).reshape(-1, 28)
+    keep = np.zeros(w.nblocks * W, bool)
+    for e in range(w.nevents):
+        b0 = int(w.event_blocks[e])
+        keep[b0 * W: b0 * W + int(w.event_tracks[e])] = True
+    r = t[keep].astype(np.float64)
+    r[:, 27] = t[keep][:, 27].copy().view(np.int32)
+    return r
+
+
+def oracle_ragged(w, fp64=False, variant="glibc"):
+    b = w.blocks
+    with O.use_variant(variant):
+        res = O.run_ragged(w.tracks, w.hits, w.nblocks, w.width, b["trk_block"], b["hit_block"], b["nlayer"],
+                           w.lane_nlayer, fp64=fp64)
+    if fp64:
+        keep = np.zeros(w.nblocks * w.width, bool)
+        for e in range(w.nevents):
+            b0 = int(w.event_blocks[e])
+            keep[b0 * w.width: b0 * w.width + int(w.event_tracks[e])] = True
+        return res[keep]
+    return res
+
+
+class Dev:
+    """A workload's arrays on the device."""
+
+    def __init__(self, ctx, w, lpt=True):
+        self.trk = ctx.upload(w.tracks)
+        self.hit = ctx.upload(w.hits)
+        self.lanes = ctx.upload(np.ascontiguousarray(w.lane_nlayer))
+        self.blk = ctx.upload(np.ascontiguousarray(w.blocks_lpt if lpt else w.blocks))
+        self.out = ctx.alloc(w.tracks.nbytes)
+
+    def free(self):
+        for b in (self.trk, self.hit, self.lanes, self.blk, self.out):
+            b.free()
+
+
+@pytest.fixture(scope="module")
+def small_ragged():
+    return p2z.Workload("ragged", nevents=12, width=32, seed=5)
+
+
+@pytest.mark.parametrize("form", ["persistent", "graph", "oneshot"])
+def test_ref_mode_bitwise_against_oracle(ctx, small_ragged, form):
+    w = small_ragged
+    log(f"case ragged-ref/{form} tracks={w.ntracks}")
+    d = Dev(ctx, w)
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "ref",
+                   persistent=form != "oneshot", graph=form == "graph")
+    out = ctx.download(d.out)
+    d.free()
+    exp = oracle_ragged(w, variant="cr")
+    eq = float(np.mean(out.view(np.uint32) == exp.view(np.uint32)))
+    log(f"  words equal {eq:.6f}")
+    assert eq >= 0.9999
+
+
+def test_widths_give_the_same_tracks(ctx):
+    """The packing is width independent (tracks sorted, then drawn), so every
+    width must produce the same per-track results."""
+    res = {}
+    for W in (16, 32, 64, 128):
+        w = p2z.Workload("ragged", nevents=6, width=W, seed=9)
+        d = Dev(ctx, w)
+        ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, W, "fast")
+        res[W] = by_track(w, ctx.download(d.out))
+        d.free()
+    for W in (16, 64, 128):
+        assert np.array_equal(res[W], res[32]), W
+
+
+def test_ragged_equals_plain_path_on_uniform_events(ctx):
+    """A uniform (precision) workload through the ragged kernel equals the
+    plain fused kernel bit for bit."""
+    w = p2z.Workload("precision", nevents=4, width=32, seed=3, ntracks=960, nlayer=20)
+    d = Dev(ctx, w, lpt=False)
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, 32, "fast")
+    a = ctx.download(d.out)
+    ctx.run(d.trk, d.hit, d.out, 4, 960 // 32, 20, 32, "fast")
+    b = ctx.download(d.out)
+    d.free()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_graph_replays_equal_eager(ctx, small_ragged):
+    """Captured graph (counter reset kernel + persistent kernel) replayed
+    several times with other work in between: every replay equals the eager
+    launch."""
+    w = small_ragged
+    d = Dev(ctx, w)
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast")
+    eager = ctx.download(d.out)
+    scratch = ctx.alloc(w.tracks.nbytes)
+    for rep in range(4):
+        ctx.run_ragged(d.trk, d.hit, scratch, d.lanes, d.blk, w.nblocks, w.width, "ref")  # other work
+        ctx.h2d(d.out, np.zeros_like(eager))
+        ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast", graph=True)
+        got = ctx.download(d.out)
+        assert np.array_equal(got.view(np.uint32), eager.view(np.uint32)), rep
+    scratch.free()
+    d.free()
+
+
+@pytest.mark.parametrize("opts", [dict(prefetch=2), dict(tracks_per_wg=64), dict(waves_per_cu=2),
+                                  dict(tracks_per_wg=128, prefetch=2, waves_per_cu=16)])
+def test_launch_shapes_are_bitwise_equal(ctx, small_ragged, opts):
+    w = small_ragged
+    d = Dev(ctx, w)
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast")
+    a = ctx.download(d.out)
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast", **opts)
+    b = ctx.download(d.out)
+    d.free()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("kind", ["ragged", "precision"])
+def test_accuracy_against_fp64(ctx, kind):
+    """Error against the FP64 oracle: the FAST path is within 2x the reference
+    FP32 algorithm's own error (the float oracle against FP64); the FP64-gain
+    path is no worse than the FP32 reference and improves the covariance."""
+    if kind == "ragged":
+        w = p2z.Workload("ragged", nevents=6, width=32, seed=21)
+    else:
+        w = p2z.Workload("precision", nevents=2, width=32, seed=21, ntracks=2048, nlayer=20)
+    o64 = oracle_ragged(w, fp64=True)
+    o32 = by_track(w, oracle_ragged(w))
+    base = compare(o32, o64)
+    d = Dev(ctx, w)
+    res = {}
+    for math in ("fast", "fp64gain"):
+        ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, math)
+        res[math] = compare(by_track(w, ctx.download(d.out)), o64)
+    d.free()
+    log(f"  {kind}: reference fp32 vs fp64 {base}")
+    for m, st in res.items():
+        log(f"  {kind}: {m} vs fp64 {st}")
+    f, g = res["fast"], res["fp64gain"]
+    assert f["nonfinite_mismatch"] <= max(2, base["nonfinite_mismatch"] * 2)
+    assert f["par_median"] <= 2 * base["par_median"] + 1e-9
+    assert f["cov_median"] <= 2 * base["cov_median"] + 1e-9
+    assert g["par_median"] <= base["par_median"] * 1.01 + 1e-9
+    assert g["cov_median"] <= base["cov_median"]
