@@ -1,17 +1,28 @@
 """Throughput of the p2z propagate-to-z + Kalman-update path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--math fast|ref|fp64gain]
+                    [--sections main,ragged,serving,precision]
 
-Workload: the reference's full configuration, 100 events x 9600 tracks x 20
-layers, AoSoA bsize 32, FP32, inputs from the reference generator (preset
-'ref': src/input_track.h, srand(1), smear 1e-7).  One step is one pass of the
-reference driver loop (src/propagate-toz-test_OMP.cpp:733-747) over all
-events: a single fused launch with tracks, hits and output resident in HBM.
+Headline workload (value): the reference's full configuration, 100 events x
+9600 tracks x 20 layers, AoSoA bsize 32, FP32, inputs from the reference
+generator (preset 'ref': src/input_track.h, srand(1), smear 1e-7).  One step
+is one pass of the reference driver loop (src/propagate-toz-test_OMP.cpp:
+733-747) over all events: a single fused launch with tracks, hits and output
+resident in HBM.
 
-Prints one JSON line: value = track-layer updates/s, a roofline object for
-the fused kernel (algorithmic HBM bytes per launch / hipEvent launch time) and
-the CPU baseline: the reference program compiled from its own source
-(oracle/_ref, OpenMP over events) on a bounded sample of the same workload.
+One JSON line: value = track-layer updates/s, a roofline object for the fused
+kernel (algorithmic HBM bytes per launch / hipEvent launch time), the CPU
+baseline (the reference program compiled from its own source, oracle/_ref,
+OpenMP over events, on a bounded sample of the same workload), and extra
+sections measured on synthetic workloads of the library's generator:
+  ragged    : 2000 uneven events (log-normal track counts, 8-26 layers per
+              track), FP32 with the FP64 gain steps; the autotuned persistent
+              launch replayed from a hipGraph against one plain launch per event
+  serving   : the same events arriving as a Poisson stream at 1k..20k events/s
+              through the continuous-batching server (HBM budget 64 GB):
+              p50 / p99 / p999 event latency and sustained throughput
+  precision : 100 events x 9600 tracks x 20 layers with wide angle / momentum /
+              hit-error ranges: FP32 path against the FP64-gain path
 """
 from __future__ import annotations
 
@@ -107,6 +118,92 @@ def cpu_baseline_port(nevts: int, ntrks: int, nlayer: int) -> dict | None:
                       f"{threads} threads over events, {secs:.3f} s"}
 
 
+def time_ms(ctx, fn, reps: int) -> float:
+    """Average hipEvent time of fn() over reps calls (after one warm call)."""
+    fn()
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(reps):
+        fn()
+    return ctx.timer_stop() / reps
+
+
+def section_ragged(ctx, nevents: int, sample_events: int) -> tuple[dict, dict]:
+    """cfg 3: autotune on a sample, then the full event set through the
+    persistent + hipGraph launch and through one plain launch per event."""
+    sys.path.insert(0, os.path.join(HERE, "p2z-tests_amd"))
+    import autotune
+
+    log(f"ragged: autotuning on {sample_events} events")
+    tuned = autotune.tune(ctx, "ragged", nevents=sample_events, math="fp64gain", reps=3)
+    best = tuned["best"]
+    W = best["width"]
+    opts = dict(tracks_per_wg=best["tracks_per_wg"], prefetch=best["prefetch"], waves_per_cu=best["waves_per_cu"])
+    log(f"ragged: best {best}; generating {nevents} events at width {W}")
+    w = p2z.Workload("ragged", nevents=nevents, width=W, seed=2024)
+    d = dict(trk=ctx.upload(w.tracks), hit=ctx.upload(w.hits), lanes=ctx.upload(np.ascontiguousarray(w.lane_nlayer)),
+             lpt=ctx.upload(np.ascontiguousarray(w.blocks_lpt)), evt=ctx.upload(np.ascontiguousarray(w.blocks)))
+    out = ctx.alloc(w.tracks.nbytes)
+    math = "fp64gain"
+    res = {"workload": f"{nevents} events, {w.ntracks} tracks, {w.track_layers} track-layers "
+                       f"(tracks/event log-normal median 2000 sigma 1 in [50,20000], layers U{{8..26}})",
+           "math": math, "tuned": {"width": W, **opts, "candidates": tuned["candidates"],
+                                   "sample_events": sample_events}}
+    ms = time_ms(ctx, lambda: ctx.run_ragged(d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, math,
+                                             graph=True, **opts), 5)
+    res["persistent_graph"] = {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3)}
+    ms = time_ms(ctx, lambda: ctx.run_ragged(d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, "fast",
+                                             graph=True, **opts), 5)
+    res["persistent_graph_fp32"] = {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3)}
+    eb = w.event_blocks
+
+    def per_event():
+        for e in range(nevents):
+            b0, b1 = int(eb[e]), int(eb[e + 1])
+            ctx.run_ragged(d["trk"], d["hit"], out, d["lanes"], d["evt"].ptr + b0 * 32, b1 - b0, W, math,
+                           persistent=False, tracks_per_wg=opts["tracks_per_wg"])
+
+    ms = time_ms(ctx, per_event, 1)
+    res["one_launch_per_event"] = {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3),
+                                   "launches": nevents}
+    res["speedup_vs_per_event"] = round(res["persistent_graph"]["tl_per_s"] / res["one_launch_per_event"]["tl_per_s"], 2)
+    out.free()
+    return res, dict(w=w, d=d, W=W, opts=opts)
+
+
+def section_serving(ctx, pool: dict, rates, budget_gb: float) -> list:
+    """cfg 4: Poisson arrivals of the resident events through the server."""
+    w, d, W, opts = pool["w"], pool["d"], pool["W"], pool["opts"]
+    rows = []
+    with p2z.Server(ctx, d["trk"], d["hit"], d["lanes"], int(budget_gb * 2**30), W, "fp64gain", nstreams=4,
+                    **opts) as sv:
+        for rate in rates:
+            log(f"serving: {rate:.0f} events/s")
+            rep, lat, ob = sv.poisson(w.blocks, w.event_blocks, rate=rate, seed=int(rate), release=True)
+            rows.append({"rate_offered": rate, "events": int(rep.nevents),
+                         "tl_per_s": w.track_layers / rep.t_span,
+                         "p50_ms": rep.lat_p50 * 1e3, "p99_ms": rep.lat_p99 * 1e3, "p999_ms": rep.lat_p999 * 1e3,
+                         "mean_ms": rep.lat_mean * 1e3, "batches": int(rep.batches),
+                         "mean_batch_events": round(rep.mean_batch_events, 2),
+                         "max_inflight_events": int(rep.max_inflight_events)})
+    return rows
+
+
+def section_precision(ctx, nevents: int) -> dict:
+    """cfg 2 throughput: the FP32 path and the FP64-gain path on the wide
+    (ill-conditioned) workload; the error report lives in the tests."""
+    w = p2z.Workload("precision", nevents=nevents, width=32, seed=42, ntracks=9600, nlayer=20)
+    trk, hit, out = ctx.upload(w.tracks), ctx.upload(w.hits), ctx.alloc(w.tracks.nbytes)
+    res = {"workload": f"{nevents} events x 9600 tracks x 20 layers, 1/pT log-U[0.01,2], theta U[0.1,3.04], "
+                       f"hit variances log-U[1e-8,1e-3]"}
+    for math in ("fast", "fp64gain"):
+        ms = time_ms(ctx, lambda: ctx.run(trk, hit, out, nevents, 9600 // 32, 20, 32, math), 20)
+        res[math] = {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3)}
+    for b in (trk, hit, out):
+        b.free()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,7 +216,13 @@ def main() -> int:
     ap.add_argument("--bsize", type=int, default=32)
     ap.add_argument("--tracks-per-wg", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sections", default="main,ragged,serving,precision",
+                    help="comma list of main, ragged, serving, precision")
+    ap.add_argument("--ragged-events", type=int, default=2000)
+    ap.add_argument("--serving-rates", default="1000,2000,5000,10000,20000")
+    ap.add_argument("--budget-gb", type=float, default=64.0)
     a = ap.parse_args()
+    sections = set(a.sections.split(","))
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path shards by event with no exchange step, and "
               "this build runs on one GPU only (--gpus 1).", file=sys.stderr)
@@ -158,6 +261,26 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(a.math),
                 "bytes_per_track_layer": b_tl, "launch_ms": round(ms_per_step, 5)}
 
+    extra = {}
+    if "ragged" in sections or "serving" in sections:
+        ctx = p2z.Context(0)
+        rag, pool = section_ragged(ctx, a.ragged_events, min(200, a.ragged_events))
+        if "ragged" in sections:
+            extra["ragged"] = rag
+        if "serving" in sections:
+            rates = [float(r) for r in a.serving_rates.split(",") if r]
+            extra["serving"] = {"budget_gb": a.budget_gb, "events_per_rate": a.ragged_events, "math": "fp64gain",
+                                "rates": section_serving(ctx, pool, rates, a.budget_gb)}
+        for b in pool["d"].values():
+            b.free()
+        pool["w"].close()
+        ctx.close()
+    if "precision" in sections:
+        ctx = p2z.Context(0)
+        log("precision workload")
+        extra["precision"] = section_precision(ctx, 100)
+        ctx.close()
+
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline")
@@ -187,6 +310,7 @@ def main() -> int:
     }
     if cpu:
         rec["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    rec.update(extra)
     print(json.dumps(rec), flush=True)
     return 0
 

@@ -140,8 +140,9 @@ int p2z_run_host(p2z_ctx* ctx, const void* trk, const void* hit, void* out, int6
  * lane_nlayer holds each track's layer count in MP1I layout (block b lane l at
  * b*W + l); a lane stops after its own count. */
 typedef struct {
-  int64_t trk_block; /* MPTRK block (input and output index)              */
+  int64_t trk_block; /* input MPTRK block (and lane_nlayer block)         */
   int64_t hit_block; /* first of the block's MPHIT blocks                 */
+  int64_t out_block; /* output MPTRK block                                */
   int32_t nlayer;    /* layer slots of the block (max of its lanes)       */
   int32_t event;     /* owning event                                      */
 } p2z_block_desc;
@@ -219,6 +220,68 @@ const int32_t* p2z_workload_event_tracks(const p2z_workload* w);
  * arrays above; load verifies the checksums. */
 int p2z_workload_save(const p2z_workload* w, const char* path);
 int p2z_workload_load(const char* path, p2z_workload** out);
+
+/* ---- serving: continuous batching of events ---------------------------------
+ * No counterpart in the reference (a batch program).  The inputs of all events
+ * are resident on the device (one ragged pool: trk, hit, lane_nlayer); a
+ * request names one event by its block descriptors (host memory; out_block
+ * counted from 0 within the event).  Admission is bounded by an output arena
+ * of budget_bytes in HBM: a request is admitted when its result blocks fit,
+ * otherwise p2z_server_submit returns P2Z_EBUSY (retry after releasing
+ * results) or P2Z_EINVAL (it can never fit).  p2z_server_pump launches every
+ * admitted request not yet running as one batch on a free stream (one
+ * persistent launch, descriptors longest-first), so requests arriving while
+ * others run are admitted as slots and memory free up and their work
+ * overlaps on the device; it also collects finished batches.  Results stay
+ * in the arena until p2z_server_release.  Per-lane arithmetic does not depend
+ * on the batch: a request's result is the same alone and in any mix.
+ * A server is used by one host thread at a time. */
+typedef struct {
+  int64_t budget_bytes;     /* output arena size                               */
+  int32_t nstreams;         /* batches in flight at once, 1..16; 0 = 4         */
+  int32_t max_batch_blocks; /* block limit of one batch; 0 = none              */
+  int32_t width, math;      /* AoSoA width of the pool, arithmetic mode        */
+  int32_t tracks_per_wg, prefetch, waves_per_cu, reserved; /* as p2z_ragged_opts */
+} p2z_server_opts;
+
+typedef struct {
+  int64_t request;    /* the caller's id                                    */
+  int64_t out_block;  /* first arena block of the result                    */
+  int64_t nblocks;    /* result blocks                                      */
+  double t_arrival, t_admit, t_launch, t_done; /* seconds, p2z_now() clock  */
+} p2z_completion;
+
+typedef struct p2z_server p2z_server;
+double p2z_now(void); /* monotonic host clock, seconds */
+int p2z_server_create(p2z_ctx* ctx, const void* trk, const void* hit, const int32_t* lane_nlayer,
+                      const p2z_server_opts* opts, p2z_server** out);
+int p2z_server_destroy(p2z_server* s); /* waits for running batches */
+int p2z_server_submit(p2z_server* s, int64_t request, const p2z_block_desc* blocks, int64_t nblocks,
+                      double t_arrival);
+int p2z_server_pump(p2z_server* s);                      /* returns completions ready  */
+int p2z_server_poll(p2z_server* s, p2z_completion* out, int32_t cap); /* returns count */
+int p2z_server_release(p2z_server* s, int64_t request);  /* frees its arena blocks     */
+const void* p2z_server_arena(const p2z_server* s);       /* device base of the arena   */
+
+typedef struct {
+  int64_t nevents, track_layers, batches, max_inflight_events, busy_rejections;
+  double rate_offered, rate_achieved;        /* events/s                        */
+  double t_span;                             /* first arrival to last done, s   */
+  double tl_per_s;                           /* track-layers / t_span           */
+  double lat_p50, lat_p99, lat_p999, lat_mean, lat_max; /* seconds            */
+  double mean_batch_events;
+} p2z_serve_report;
+
+/* Poisson arrivals at `rate` events/s of the pool's events (event e = blocks
+ * [event_blocks[e], event_blocks[e+1]) of the event-ordered descriptor table),
+ * submitted as they arrive, admitted as the arena allows, pumped until all
+ * are done.  release != 0 frees each result when done (steady state);
+ * otherwise results stay (the arena must hold them all).  latency: optional
+ * per-event done - arrival (nevents doubles); out_block: optional per-event
+ * arena position of the result. */
+int p2z_serve_poisson(p2z_server* s, const p2z_block_desc* blocks, const int64_t* event_blocks,
+                      int64_t nevents, double rate, uint64_t seed, int32_t release, p2z_serve_report* rep,
+                      double* latency, int64_t* out_block);
 
 /* ---- report (device) ------------------------------------------------------
  * The statistics of OMP:758-876 computed on the GPU from device-resident

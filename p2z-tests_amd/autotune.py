@@ -1,0 +1,97 @@
+"""Autotuner of the persistent ragged launch.
+
+Sweeps the AoSoA width W in {16, 32, 64, 128}, tracks per workgroup, hit
+prefetch depth and the persistent grid size (waves per CU) on a sample of the
+workload, timing each candidate with hipEvents (median of repeats), and
+returns the fastest.  The sample is generated at each width by the same
+generator (the tracks do not depend on W, so every candidate does the same
+work).  Results are cached per device and workload kind in
+~/.cache/p2z/autotune.json unless cache=False.
+
+    python autotune.py [--kind ragged] [--events 200]
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import p2z  # noqa: E402
+
+WIDTHS = (16, 32, 64, 128)
+TPWS = (64, 128, 256)
+PREFETCH = (1, 2)
+WAVES_PER_CU = (4, 8, 16)
+CACHE = os.path.join(os.path.expanduser("~"), ".cache", "p2z", "autotune.json")
+
+
+def _time(ctx, fn, reps: int) -> float:
+    fn()
+    ctx.sync()
+    ts = []
+    for _ in range(reps):
+        ctx.timer_start()
+        fn()
+        ts.append(ctx.timer_stop())
+    return float(np.median(ts))
+
+
+def tune(ctx, kind: str = "ragged", nevents: int = 200, math: str = "fp64gain", seed: int = 17,
+         reps: int = 5, widths=WIDTHS, tpws=TPWS, prefetch=PREFETCH, waves=WAVES_PER_CU,
+         log=lambda s: None) -> dict:
+    results = []
+    for W in widths:
+        w = p2z.Workload(kind, nevents=nevents, width=W, seed=seed)
+        trk, hit = ctx.upload(w.tracks), ctx.upload(w.hits)
+        lanes = ctx.upload(np.ascontiguousarray(w.lane_nlayer))
+        blk = ctx.upload(np.ascontiguousarray(w.blocks_lpt))
+        out = ctx.alloc(w.tracks.nbytes)
+        for tpw, pf, wpc in itertools.product(tpws, prefetch, waves):
+            ms = _time(ctx, lambda: ctx.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, math,
+                                                   tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc), reps)
+            rec = dict(width=W, tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc, ms=ms,
+                       tl_per_s=w.track_layers / (ms * 1e-3))
+            results.append(rec)
+            log(f"  W={W:3d} tpw={tpw:3d} prefetch={pf} waves/CU={wpc:2d}: {ms:.4f} ms  {rec['tl_per_s']:.3e} TL/s")
+        for b in (trk, hit, lanes, blk, out):
+            b.free()
+        w.close()
+    best = min(results, key=lambda r: r["ms"])
+    return dict(best=best, kind=kind, math=math, sample_events=nevents, candidates=len(results),
+                results=results)
+
+
+def cached_or_tune(ctx, key: str, cache: bool = True, **kw) -> dict:
+    db = {}
+    if cache and os.path.exists(CACHE):
+        with open(CACHE) as f:
+            db = json.load(f)
+    if key in db:
+        return db[key]
+    res = tune(ctx, **kw)
+    if cache:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        db[key] = res
+        with open(CACHE, "w") as f:
+            json.dump(db, f, indent=1)
+    return res
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--kind", default="ragged", choices=list(p2z.WORKLOAD))
+    ap.add_argument("--events", type=int, default=200)
+    ap.add_argument("--math", default="fp64gain", choices=list(p2z.MATH))
+    a = ap.parse_args()
+    with p2z.Context(0) as ctx:
+        res = tune(ctx, a.kind, a.events, a.math, log=print)
+    print(json.dumps(res["best"]))
+
+
+if __name__ == "__main__":
+    main()

@@ -13,36 +13,21 @@
 #include "p2z.h"
 #include "p2z_capi_util.hpp"
 #include "p2z_internal.hpp"
+#include "p2z_ctx.hpp"
 
 using p2z::LaunchCfg;
 
-struct p2z_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;            // main stream: device-pointer entry points
-  std::vector<hipStream_t> pipe;           // host-buffer pipeline streams
-  hipEvent_t t0 = nullptr, t1 = nullptr;   // timer
-  LaunchCfg cfg;
-  // host-buffer pipeline staging, one slot per pipeline stream
-  std::vector<void*> slot_trk, slot_hit, slot_out;
-  size_t slot_trk_bytes = 0, slot_hit_bytes = 0;
-  // device report scratch
-  double* sum_part = nullptr;
-  size_t sum_part_len = 0;
-  double* sum_res = nullptr;
-  // ragged path: work-queue counter, compute units, captured graphs
-  unsigned* counter = nullptr;
-  int cus = 0;
-  using GraphKey = std::tuple<const void*, const void*, void*, const void*, const void*, int64_t, int, int, int,
-                              int, int64_t>;
-  std::map<GraphKey, hipGraphExec_t> graphs;
-};
 
-namespace {
 
+namespace p2z {
 int use_device(p2z_ctx* c) {
   P2Z_HIP_CHECK(hipSetDevice(c->device));
   return P2Z_OK;
 }
+}  // namespace p2z
+
+namespace {
+using p2z::use_device;
 
 int apply_opts(p2z_ctx* c, const p2z_opts* o) {
   LaunchCfg cfg;

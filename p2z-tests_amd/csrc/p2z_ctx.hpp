@@ -1,0 +1,35 @@
+// The p2z context (opaque in the C ABI); shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "p2z_internal.hpp"
+
+struct p2z_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;            // main stream: device-pointer entry points
+  std::vector<hipStream_t> pipe;           // host-buffer pipeline streams
+  hipEvent_t t0 = nullptr, t1 = nullptr;   // timer
+  p2z::LaunchCfg cfg;
+  // host-buffer pipeline staging, one slot per pipeline stream
+  std::vector<void*> slot_trk, slot_hit, slot_out;
+  size_t slot_trk_bytes = 0, slot_hit_bytes = 0;
+  // device report scratch
+  double* sum_part = nullptr;
+  size_t sum_part_len = 0;
+  double* sum_res = nullptr;
+  // ragged path: work-queue counter, compute units, captured graphs
+  unsigned* counter = nullptr;
+  int cus = 0;
+  using GraphKey = std::tuple<const void*, const void*, void*, const void*, const void*, int64_t, int, int, int,
+                              int, int64_t>;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+};
+
+namespace p2z {
+// Makes the context's device current.
+int use_device(p2z_ctx* c);
+}  // namespace p2z

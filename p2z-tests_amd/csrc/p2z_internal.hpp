@@ -35,10 +35,11 @@ hipError_t launch_update(float* trkErr, float* inPar, const float* hitErr, const
 struct BlockDesc {
   int64_t trk_block;
   int64_t hit_block;
+  int64_t out_block;
   int32_t nlayer;
   int32_t event;
 };
-static_assert(sizeof(BlockDesc) == 24, "descriptor layout");
+static_assert(sizeof(BlockDesc) == 32, "descriptor layout");
 
 // Ragged pass.  persistent: grid workgroups drain the descriptor table through
 // the device counter (reset first on the same stream); otherwise one wave per

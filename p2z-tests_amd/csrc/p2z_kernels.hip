@@ -106,7 +106,7 @@ __device__ inline void ragged_track(const float* __restrict__ trk, const float* 
       layer_step<kMath, false>(t, h0);
     }
   }
-  store_track<W>(t, out + d.trk_block * (kTrkWords * W) + l);
+  store_track<W>(t, out + d.out_block * (kTrkWords * W) + l);
 }
 
 // Work item = 64 consecutive track slots of the descriptor table (one wave):

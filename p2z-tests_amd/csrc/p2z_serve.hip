@@ -1,0 +1,386 @@
+// Serving loop: admission by an HBM output arena, continuous batching of
+// events into persistent launches on several streams, per-event timestamps
+// (declared in include/p2z.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <map>
+#include <random>
+#include <unordered_map>
+#include <vector>
+
+#include "p2z.h"
+#include "p2z_capi_util.hpp"
+#include "p2z_ctx.hpp"
+#include "p2z_internal.hpp"
+
+namespace {
+
+// First-fit allocator of arena blocks (ranges start -> length).
+class Arena {
+ public:
+  explicit Arena(int64_t n) : total_(n) {
+    if (n > 0) free_[0] = n;
+  }
+  int64_t alloc(int64_t n) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        const int64_t at = it->first, rest = it->second - n;
+        free_.erase(it);
+        if (rest > 0) free_[at + n] = rest;
+        used_ += n;
+        return at;
+      }
+    }
+    return -1;
+  }
+  void release(int64_t at, int64_t n) {
+    auto it = free_.emplace(at, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && at + n == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+    used_ -= n;
+  }
+  int64_t total() const { return total_; }
+  int64_t used() const { return used_; }
+
+ private:
+  std::map<int64_t, int64_t> free_;
+  int64_t total_ = 0, used_ = 0;
+};
+
+struct Request {
+  std::vector<p2z_block_desc> blocks;  // out_block rewritten to arena positions
+  int64_t out_block = 0, nblocks = 0;
+  double t_arrival = 0, t_admit = 0, t_launch = 0, t_done = 0;
+  bool done = false;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  unsigned* counter = nullptr;
+  p2z::BlockDesc* d_desc = nullptr;
+  p2z_block_desc* h_desc = nullptr;  // pinned staging
+  int64_t cap = 0;
+  std::vector<int64_t> requests;
+  bool busy = false;
+};
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct p2z_server {
+  p2z_ctx* ctx = nullptr;
+  p2z_server_opts opts{};
+  const float* trk = nullptr;
+  const float* hit = nullptr;
+  const int32_t* lanes = nullptr;
+  float* arena_mem = nullptr;
+  Arena* arena = nullptr;
+  std::unordered_map<int64_t, Request> reqs;
+  std::deque<int64_t> pending;
+  std::vector<Slot> slots;
+  std::vector<p2z_completion> ready;
+  int64_t batches = 0, batch_events = 0;
+};
+
+namespace {
+
+int slot_reserve(Slot& s, int64_t n) {
+  if (s.cap >= n) return P2Z_OK;
+  if (s.d_desc) P2Z_HIP_CHECK(hipFree(s.d_desc));
+  if (s.h_desc) P2Z_HIP_CHECK(hipHostFree(s.h_desc));
+  s.d_desc = nullptr;
+  s.h_desc = nullptr;
+  s.cap = 0;
+  const int64_t cap = std::max<int64_t>(n, 1024);
+  P2Z_HIP_CHECK(hipMalloc(&s.d_desc, cap * sizeof(p2z::BlockDesc)));
+  P2Z_HIP_CHECK(hipHostMalloc(&s.h_desc, cap * sizeof(p2z_block_desc), hipHostMallocDefault));
+  s.cap = cap;
+  return P2Z_OK;
+}
+
+int harvest(p2z_server* sv) {
+  for (auto& s : sv->slots) {
+    if (!s.busy) continue;
+    const hipError_t q = hipEventQuery(s.done);
+    if (q == hipErrorNotReady) continue;
+    P2Z_HIP_CHECK(q);
+    const double t = now_s();
+    for (int64_t id : s.requests) {
+      Request& r = sv->reqs.at(id);
+      r.done = true;
+      r.t_done = t;
+      sv->ready.push_back(p2z_completion{id, r.out_block, r.nblocks, r.t_arrival, r.t_admit, r.t_launch, r.t_done});
+    }
+    s.requests.clear();
+    s.busy = false;
+  }
+  return P2Z_OK;
+}
+
+int launch_batches(p2z_server* sv) {
+  const p2z_server_opts& o = sv->opts;
+  for (auto& s : sv->slots) {
+    if (sv->pending.empty()) break;
+    if (s.busy) continue;
+    // every admitted request waiting, up to the block limit (at least one)
+    std::vector<int64_t> ids;
+    int64_t nblk = 0;
+    while (!sv->pending.empty()) {
+      const Request& r = sv->reqs.at(sv->pending.front());
+      if (!ids.empty() && o.max_batch_blocks > 0 && nblk + r.nblocks > o.max_batch_blocks) break;
+      ids.push_back(sv->pending.front());
+      nblk += r.nblocks;
+      sv->pending.pop_front();
+    }
+    P2Z_TRY(slot_reserve(s, nblk));
+    int64_t k = 0;
+    for (int64_t id : ids)
+      for (const auto& d : sv->reqs.at(id).blocks) s.h_desc[k++] = d;
+    std::stable_sort(s.h_desc, s.h_desc + nblk,
+                     [](const p2z_block_desc& a, const p2z_block_desc& b) { return a.nlayer > b.nlayer; });
+    const int tpw = o.tracks_per_wg ? o.tracks_per_wg : sv->ctx->cfg.tracks_per_wg;
+    const int wpc = o.waves_per_cu ? o.waves_per_cu : 8;
+    const int64_t grid = ((int64_t)sv->ctx->cus * wpc * 64 + tpw - 1) / tpw;
+    P2Z_HIP_CHECK(hipMemcpyAsync(s.d_desc, s.h_desc, nblk * sizeof(p2z_block_desc), hipMemcpyHostToDevice, s.stream));
+    P2Z_HIP_CHECK(p2z::launch_ragged(sv->trk, sv->hit, sv->arena_mem, sv->lanes, s.d_desc, nblk, o.width, o.math,
+                                     true, o.prefetch ? o.prefetch : 1, tpw, grid, s.counter, s.stream));
+    P2Z_HIP_CHECK(hipEventRecord(s.done, s.stream));
+    const double t = now_s();
+    for (int64_t id : ids) sv->reqs.at(id).t_launch = t;
+    s.requests = std::move(ids);
+    s.busy = true;
+    sv->batches += 1;
+    sv->batch_events += (int64_t)s.requests.size();
+  }
+  return P2Z_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+double p2z_now(void) { return now_s(); }
+
+int p2z_server_create(p2z_ctx* ctx, const void* trk, const void* hit, const int32_t* lane_nlayer,
+                      const p2z_server_opts* opts, p2z_server** out) {
+  if (!ctx || !trk || !hit || !lane_nlayer || !opts || !out) return p2z::fail(P2Z_EINVAL, "null argument");
+  *out = nullptr;
+  p2z_server_opts o = *opts;
+  if (!p2z::valid_width(o.width)) return p2z::fail(P2Z_EINVAL, "width must be 16/32/64/128 (got %d)", o.width);
+  if (!p2z::valid_math(o.math)) return p2z::fail(P2Z_EINVAL, "unknown math mode %d", o.math);
+  if (o.nstreams == 0) o.nstreams = 4;
+  if (o.nstreams < 1 || o.nstreams > 16) return p2z::fail(P2Z_EINVAL, "nstreams must be in [1,16]");
+  if (o.max_batch_blocks < 0) return p2z::fail(P2Z_EINVAL, "max_batch_blocks must be >= 0");
+  if (o.tracks_per_wg && (o.tracks_per_wg < 64 || o.tracks_per_wg > 256 || o.tracks_per_wg % 64))
+    return p2z::fail(P2Z_EINVAL, "tracks_per_wg must be 64, 128, 192 or 256");
+  if (o.prefetch < 0 || o.prefetch > 2) return p2z::fail(P2Z_EINVAL, "prefetch must be 0, 1 or 2");
+  if (o.waves_per_cu < 0 || o.waves_per_cu > 32) return p2z::fail(P2Z_EINVAL, "waves_per_cu in [0,32]");
+  const int64_t blk_bytes = (int64_t)p2z::kTrkWords * o.width * sizeof(float);
+  const int64_t nblk = o.budget_bytes / blk_bytes;
+  if (nblk <= 0) return p2z::fail(P2Z_EINVAL, "budget of %lld bytes holds no result block", (long long)o.budget_bytes);
+  P2Z_TRY(p2z::use_device(ctx));
+  auto* sv = new p2z_server;
+  sv->ctx = ctx;
+  sv->opts = o;
+  sv->trk = static_cast<const float*>(trk);
+  sv->hit = static_cast<const float*>(hit);
+  sv->lanes = lane_nlayer;
+  int rc = P2Z_OK;
+  const hipError_t e = hipMalloc(&sv->arena_mem, nblk * blk_bytes);
+  if (e == hipErrorOutOfMemory) rc = p2z::fail(P2Z_ENOMEM, "arena of %lld bytes does not fit", (long long)(nblk * blk_bytes));
+  else rc = p2z::hip_status(e, "hipMalloc");
+  if (rc == P2Z_OK) {
+    sv->arena = new Arena(nblk);
+    sv->slots.resize(o.nstreams);
+    for (auto& s : sv->slots) {
+      if (rc == P2Z_OK) rc = p2z::hip_status(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+      if (rc == P2Z_OK) rc = p2z::hip_status(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate");
+      if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc(&s.counter, 256), "hipMalloc");
+    }
+  }
+  if (rc != P2Z_OK) {
+    p2z_server_destroy(sv);
+    return rc;
+  }
+  *out = sv;
+  return P2Z_OK;
+}
+
+int p2z_server_destroy(p2z_server* sv) {
+  if (!sv) return P2Z_OK;
+  if (sv->ctx) (void)p2z::use_device(sv->ctx);
+  for (auto& s : sv->slots) {
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.d_desc) (void)hipFree(s.d_desc);
+    if (s.h_desc) (void)hipHostFree(s.h_desc);
+    if (s.counter) (void)hipFree(s.counter);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+  }
+  if (sv->arena_mem) (void)hipFree(sv->arena_mem);
+  delete sv->arena;
+  delete sv;
+  return P2Z_OK;
+}
+
+int p2z_server_submit(p2z_server* sv, int64_t request, const p2z_block_desc* blocks, int64_t nblocks,
+                      double t_arrival) {
+  if (!sv || (!blocks && nblocks)) return p2z::fail(P2Z_EINVAL, "null argument");
+  if (nblocks <= 0) return p2z::fail(P2Z_EINVAL, "a request needs at least one block");
+  if (sv->reqs.count(request)) return p2z::fail(P2Z_EINVAL, "request %lld already submitted", (long long)request);
+  if (nblocks > sv->arena->total())
+    return p2z::fail(P2Z_EINVAL, "request %lld needs %lld result blocks, the arena holds %lld", (long long)request,
+                     (long long)nblocks, (long long)sv->arena->total());
+  for (int64_t i = 0; i < nblocks; ++i)
+    if (blocks[i].out_block < 0 || blocks[i].out_block >= nblocks || blocks[i].nlayer < 0 || blocks[i].trk_block < 0 ||
+        blocks[i].hit_block < 0)
+      return p2z::fail(P2Z_EINVAL, "request %lld: bad descriptor %lld", (long long)request, (long long)i);
+  const int64_t at = sv->arena->alloc(nblocks);
+  if (at < 0) return p2z::fail(P2Z_EBUSY, "arena full (%lld of %lld blocks in use)", (long long)sv->arena->used(),
+                               (long long)sv->arena->total());
+  Request r;
+  r.blocks.assign(blocks, blocks + nblocks);
+  for (auto& d : r.blocks) d.out_block += at;
+  r.out_block = at;
+  r.nblocks = nblocks;
+  r.t_arrival = t_arrival;
+  r.t_admit = now_s();
+  sv->reqs.emplace(request, std::move(r));
+  sv->pending.push_back(request);
+  return P2Z_OK;
+}
+
+int p2z_server_pump(p2z_server* sv) {
+  if (!sv) return p2z::fail(P2Z_EINVAL, "null server");
+  P2Z_TRY(p2z::use_device(sv->ctx));
+  P2Z_TRY(harvest(sv));
+  P2Z_TRY(launch_batches(sv));
+  return (int)sv->ready.size();
+}
+
+int p2z_server_poll(p2z_server* sv, p2z_completion* out, int32_t cap) {
+  if (!sv || (!out && cap > 0)) return p2z::fail(P2Z_EINVAL, "null argument");
+  const int n = std::min<int>(cap, (int)sv->ready.size());
+  for (int i = 0; i < n; ++i) out[i] = sv->ready[i];
+  sv->ready.erase(sv->ready.begin(), sv->ready.begin() + n);
+  return n;
+}
+
+int p2z_server_release(p2z_server* sv, int64_t request) {
+  if (!sv) return p2z::fail(P2Z_EINVAL, "null server");
+  auto it = sv->reqs.find(request);
+  if (it == sv->reqs.end()) return p2z::fail(P2Z_EINVAL, "unknown request %lld", (long long)request);
+  if (!it->second.done) return p2z::fail(P2Z_ESTATE, "request %lld is not finished", (long long)request);
+  sv->arena->release(it->second.out_block, it->second.nblocks);
+  sv->reqs.erase(it);
+  return P2Z_OK;
+}
+
+const void* p2z_server_arena(const p2z_server* sv) { return sv ? sv->arena_mem : nullptr; }
+
+int p2z_serve_poisson(p2z_server* sv, const p2z_block_desc* blocks, const int64_t* event_blocks, int64_t nevents,
+                      double rate, uint64_t seed, int32_t release, p2z_serve_report* rep, double* latency,
+                      int64_t* out_block) {
+  if (!sv || !blocks || !event_blocks || !rep) return p2z::fail(P2Z_EINVAL, "null argument");
+  if (nevents <= 0 || !(rate > 0)) return p2z::fail(P2Z_EINVAL, "need nevents > 0 and rate > 0");
+  if (!sv->reqs.empty()) return p2z::fail(P2Z_ESTATE, "the server must be idle");
+  // per-event descriptors with out_block counted from 0
+  std::vector<std::vector<p2z_block_desc>> ev(nevents);
+  for (int64_t e = 0; e < nevents; ++e) {
+    const int64_t b0 = event_blocks[e], b1 = event_blocks[e + 1];
+    if (b1 <= b0) return p2z::fail(P2Z_EINVAL, "event %lld has no blocks", (long long)e);
+    ev[e].assign(blocks + b0, blocks + b1);
+    for (int64_t i = 0; i < b1 - b0; ++i) ev[e][i].out_block = i;
+  }
+  std::mt19937_64 rng(seed);
+  std::exponential_distribution<double> gap(rate);
+  std::vector<double> arrive(nevents), lat(nevents, 0.0);
+  const double t0 = now_s() + 1e-3;
+  double t = t0;
+  for (int64_t e = 0; e < nevents; ++e) {
+    t += gap(rng);
+    arrive[e] = t;
+  }
+  std::deque<int64_t> backlog;
+  int64_t next = 0, ndone = 0, inflight = 0, max_inflight = 0, rejections = 0;
+  double t_last = t0;
+  std::vector<p2z_completion> comp(256);
+  sv->batches = sv->batch_events = 0;
+  while (ndone < nevents) {
+    const double now = now_s();
+    while (next < nevents && arrive[next] <= now) backlog.push_back(next++);
+    while (!backlog.empty()) {
+      const int64_t e = backlog.front();
+      const int rc = p2z_server_submit(sv, e, ev[e].data(), (int64_t)ev[e].size(), arrive[e]);
+      if (rc == P2Z_EBUSY) {
+        // nothing in flight and still no room: the kept results fill the arena
+        if (inflight == 0 && !release)
+          return p2z::fail(P2Z_ENOMEM, "the arena cannot hold the results kept so far (event %lld)", (long long)e);
+        ++rejections;
+        break;
+      }
+      P2Z_TRY(rc);
+      backlog.pop_front();
+      max_inflight = std::max(max_inflight, ++inflight);
+    }
+    const int r = p2z_server_pump(sv);
+    if (r < 0) return r;
+    int n;
+    while ((n = p2z_server_poll(sv, comp.data(), (int32_t)comp.size())) > 0) {
+      for (int i = 0; i < n; ++i) {
+        const p2z_completion& c = comp[i];
+        lat[c.request] = c.t_done - c.t_arrival;
+        if (out_block) out_block[c.request] = c.out_block;
+        t_last = std::max(t_last, c.t_done);
+        if (release) P2Z_TRY(p2z_server_release(sv, c.request));
+        --inflight;
+        ++ndone;
+      }
+    }
+  }
+  std::vector<double> s = lat;
+  std::sort(s.begin(), s.end());
+  auto q = [&](double p) { return s[(size_t)std::min<double>((double)(s.size() - 1), std::floor(p * (s.size() - 1) + 0.5))]; };
+  double mean = 0;
+  for (double v : s) mean += v;
+  mean /= (double)s.size();
+  *rep = p2z_serve_report{};
+  rep->nevents = nevents;
+  rep->track_layers = 0;  // the lane layer counts live on the device: the caller fills these two
+  rep->tl_per_s = 0;
+  rep->batches = sv->batches;
+  rep->max_inflight_events = max_inflight;
+  rep->busy_rejections = rejections;
+  rep->rate_offered = rate;
+  rep->t_span = t_last - arrive[0];
+  rep->rate_achieved = nevents / (arrive[nevents - 1] - arrive[0] > 0 ? arrive[nevents - 1] - arrive[0] : 1.0);
+  rep->lat_p50 = q(0.5);
+  rep->lat_p99 = q(0.99);
+  rep->lat_p999 = q(0.999);
+  rep->lat_mean = mean;
+  rep->lat_max = s.back();
+  rep->mean_batch_events = sv->batches ? (double)sv->batch_events / sv->batches : 0.0;
+  if (latency)
+    for (int64_t e = 0; e < nevents; ++e) latency[e] = lat[e];
+  return P2Z_OK;
+}
+
+}  // extern "C"

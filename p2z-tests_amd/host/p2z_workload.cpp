@@ -186,7 +186,7 @@ int build(const p2z_workload_spec& sp, p2z_workload& w) {
     for (int64_t b = w.event_blocks[e]; b < w.event_blocks[e + 1]; ++b) {
       const int64_t t0 = (b - w.event_blocks[e]) * W;
       const int nslot = lays[e][t0];
-      w.blocks[b] = p2z_block_desc{b, hb, nslot, (int32_t)e};
+      w.blocks[b] = p2z_block_desc{b, hb, b, nslot, (int32_t)e};
       for (int l = 0; l < W && t0 + l < n; ++l) {
         const int nl = lays[e][t0 + l];
         w.lane_nlay[(size_t)b * W + l] = nl;

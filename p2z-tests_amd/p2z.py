@@ -66,10 +66,12 @@ class Summary(C.Structure):
 
 
 class BlockDesc(C.Structure):
-    _fields_ = [("trk_block", C.c_int64), ("hit_block", C.c_int64), ("nlayer", C.c_int32), ("event", C.c_int32)]
+    _fields_ = [("trk_block", C.c_int64), ("hit_block", C.c_int64), ("out_block", C.c_int64),
+                ("nlayer", C.c_int32), ("event", C.c_int32)]
 
 
-BLOCK_DESC_DTYPE = np.dtype([("trk_block", "<i8"), ("hit_block", "<i8"), ("nlayer", "<i4"), ("event", "<i4")])
+BLOCK_DESC_DTYPE = np.dtype([("trk_block", "<i8"), ("hit_block", "<i8"), ("out_block", "<i8"), ("nlayer", "<i4"),
+                             ("event", "<i4")])
 
 
 class RaggedOpts(C.Structure):
@@ -85,6 +87,27 @@ class WorkloadSpec(C.Structure):
 class WorkloadInfo(C.Structure):
     _fields_ = [("nevents", C.c_int64), ("ntracks", C.c_int64), ("nblocks", C.c_int64), ("nhit_blocks", C.c_int64),
                 ("track_layers", C.c_int64), ("width", C.c_int32), ("kind", C.c_int32)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("budget_bytes", C.c_int64), ("nstreams", C.c_int32), ("max_batch_blocks", C.c_int32),
+                ("width", C.c_int32), ("math", C.c_int32), ("tracks_per_wg", C.c_int32), ("prefetch", C.c_int32),
+                ("waves_per_cu", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("request", C.c_int64), ("out_block", C.c_int64), ("nblocks", C.c_int64), ("t_arrival", C.c_double),
+                ("t_admit", C.c_double), ("t_launch", C.c_double), ("t_done", C.c_double)]
+
+
+class ServeReport(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("nevents", "track_layers", "batches", "max_inflight_events",
+                                          "busy_rejections")] + \
+               [(n, C.c_double) for n in ("rate_offered", "rate_achieved", "t_span", "tl_per_s", "lat_p50", "lat_p99",
+                                          "lat_p999", "lat_mean", "lat_max", "mean_batch_events")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 RAGGED_PERSISTENT, RAGGED_GRAPH = 1, 2
@@ -132,6 +155,15 @@ _SIGS = {
     "p2z_workload_event_tracks": ([vp], vp),
     "p2z_workload_save": ([vp, C.c_char_p], i32),
     "p2z_workload_load": ([C.c_char_p, C.POINTER(vp)], i32),
+    "p2z_now": ([], dbl),
+    "p2z_server_create": ([vp, vp, vp, vp, C.POINTER(ServerOpts), C.POINTER(vp)], i32),
+    "p2z_server_destroy": ([vp], i32),
+    "p2z_server_submit": ([vp, i64, vp, i64, dbl], i32),
+    "p2z_server_pump": ([vp], i32),
+    "p2z_server_poll": ([vp, C.POINTER(Completion), i32], i32),
+    "p2z_server_release": ([vp, i64], i32),
+    "p2z_server_arena": ([vp], vp),
+    "p2z_serve_poisson": ([vp, vp, vp, i64, dbl, C.c_uint64, i32, C.POINTER(ServeReport), vp, vp], i32),
     "p2z_summary_device": ([vp, vp, i64, i64, i32, i64, C.POINTER(C.c_float), C.POINTER(Summary)], i32),
     "p2z_summary_host": ([vp, i64, i64, i32, i64, C.POINTER(C.c_float), C.POINTER(Summary)], i32),
     "p2z_format_summary": ([C.POINTER(Summary), C.c_char_p, i32], i32),
@@ -351,6 +383,93 @@ class DeviceBuffer:
         if self.ptr:
             check(lib().p2z_dev_free(self.ctx.handle, self.ptr), "p2z_dev_free")
             self.ptr = 0
+
+
+class Server:
+    """Continuous-batching server over a device-resident ragged pool
+    (p2z_server_*): requests are events given by their block descriptors."""
+
+    def __init__(self, ctx: "Context", trk, hit, lane_nlayer, budget_bytes: int, width: int, math="fast",
+                 nstreams: int = 0, max_batch_blocks: int = 0, tracks_per_wg: int = 0, prefetch: int = 0,
+                 waves_per_cu: int = 0):
+        self.ctx = ctx
+        self.width = width
+        o = ServerOpts(budget_bytes, nstreams, max_batch_blocks, width, MATH.get(math, math), tracks_per_wg,
+                       prefetch, waves_per_cu, 0)
+        self.handle = vp()
+        check(lib().p2z_server_create(ctx.handle, _ptr(trk), _ptr(hit), _ptr(lane_nlayer), C.byref(o),
+                                      C.byref(self.handle)), "p2z_server_create")
+
+    def close(self) -> None:
+        if self.handle:
+            lib().p2z_server_destroy(self.handle)
+            self.handle = vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, request: int, blocks: np.ndarray, t_arrival: float | None = None) -> int:
+        """Returns OK or EBUSY; raises on other errors."""
+        blocks = np.ascontiguousarray(blocks, BLOCK_DESC_DTYPE)
+        t = lib().p2z_now() if t_arrival is None else t_arrival
+        rc = lib().p2z_server_submit(self.handle, request, blocks.ctypes.data, len(blocks), t)
+        if rc == EBUSY:
+            return rc
+        return check(rc, "p2z_server_submit")
+
+    def pump(self) -> int:
+        return check(lib().p2z_server_pump(self.handle), "p2z_server_pump")
+
+    def poll(self, cap: int = 256) -> list:
+        arr = (Completion * cap)()
+        n = check(lib().p2z_server_poll(self.handle, arr, cap), "p2z_server_poll")
+        return [arr[i] for i in range(n)]
+
+    def release(self, request: int) -> None:
+        check(lib().p2z_server_release(self.handle, request), "p2z_server_release")
+
+    def arena_ptr(self) -> int:
+        return lib().p2z_server_arena(self.handle)
+
+    def wait(self, requests, timeout: float = 60.0) -> dict:
+        """Pumps until every request in `requests` is done; returns id -> completion."""
+        want, got = set(requests), {}
+        t0 = lib().p2z_now()
+        while want - set(got):
+            self.pump()
+            for c in self.poll():
+                got[c.request] = c
+            if lib().p2z_now() - t0 > timeout:
+                raise TimeoutError(f"requests not done after {timeout} s: {sorted(want - set(got))[:5]}")
+        return got
+
+    def result(self, completion, dtype=np.float32) -> np.ndarray:
+        """Copies one finished request's result blocks from the arena."""
+        W = self.width
+        nbytes = completion.nblocks * TRK_WORDS * W * 4
+        out = np.empty(nbytes // 4, dtype)
+        src = self.arena_ptr() + completion.out_block * TRK_WORDS * W * 4
+        check(lib().p2z_memcpy_d2h(self.ctx.handle, out.ctypes.data, src, nbytes), "p2z_memcpy_d2h")
+        self.ctx.sync()
+        return out
+
+    def poisson(self, blocks: np.ndarray, event_blocks: np.ndarray, rate: float, seed: int = 1,
+                release: bool = True, nevents: int | None = None):
+        """Poisson arrivals through the native serving loop; returns
+        (ServeReport, latencies[nevents], out_block[nevents])."""
+        blocks = np.ascontiguousarray(blocks, BLOCK_DESC_DTYPE)
+        event_blocks = np.ascontiguousarray(event_blocks, np.int64)
+        n = len(event_blocks) - 1 if nevents is None else nevents
+        rep = ServeReport()
+        lat = np.zeros(n, np.float64)
+        ob = np.zeros(n, np.int64)
+        check(lib().p2z_serve_poisson(self.handle, blocks.ctypes.data, event_blocks.ctypes.data, n, rate, seed,
+                                      1 if release else 0, C.byref(rep), lat.ctypes.data, ob.ctypes.data),
+              "p2z_serve_poisson")
+        return rep, lat, ob
 
 
 class Context:

@@ -1,0 +1,67 @@
+"""Precision variant (wide-range workload): per-track relative error of the
+FP32 path and of the FP64-gain path (FP64 2x2 inverse, gain, parameter update
+and Joseph-form covariance) against the full-FP64 oracle, beside the error of
+the reference FP32 algorithm itself (the float oracle).  Prints the
+quantiles; asserts that the FP64-gain path is the more accurate one."""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+import p2z
+import pyoracle as O
+from parity import PAR_FLOOR, _DIAG, _IJ, COV_FLOOR
+
+pytestmark = pytest.mark.gpu
+
+
+def per_track_errors(a: np.ndarray, b: np.ndarray):
+    """Per-track max relative error of parameters and of covariance (tests/parity.py
+    metric), over tracks finite in b."""
+    fin = np.all(np.isfinite(b[:, :6]), axis=1)
+    a, b = a[fin], b[fin]
+    par = (np.abs(a[:, :6] - b[:, :6]) / np.maximum(np.abs(b[:, :6]), PAR_FLOOR)).max(axis=1)
+    Pb = b[:, 6:27]
+    d = np.abs(Pb[:, _DIAG])
+    scale = np.stack([np.sqrt(d[:, i] * d[:, j]) for i, j in _IJ], 1)
+    cov = (np.abs(a[:, 6:27] - Pb) / np.maximum(np.maximum(np.abs(Pb), COV_FLOOR * scale), 1e-300)).max(axis=1)
+    par = np.where(np.isfinite(par), par, np.inf)
+    cov = np.where(np.isfinite(cov), cov, np.inf)
+    return par, cov, int((~fin).sum())
+
+
+def quantiles(x: np.ndarray) -> dict:
+    q = {"median": float(np.median(x)), "p90": float(np.quantile(x, 0.9)), "p99": float(np.quantile(x, 0.99)),
+         "p999": float(np.quantile(x, 0.999)), "max": float(x.max())}
+    q["frac_le_1e-5"] = float(np.mean(x <= 1e-5))
+    return q
+
+
+def test_precision_report(ctx):
+    nevents, ntrk = 4, 9600
+    w = p2z.Workload("precision", nevents=nevents, width=32, seed=42, ntracks=ntrk, nlayer=20)
+    b = w.blocks
+    o64 = O.run_ragged(w.tracks, w.hits, w.nblocks, 32, b["trk_block"], b["hit_block"], b["nlayer"],
+                       w.lane_nlayer, fp64=True)
+    o32 = O.run_ragged(w.tracks, w.hits, w.nblocks, 32, b["trk_block"], b["hit_block"], b["nlayer"], w.lane_nlayer)
+    from test_gpu_ragged import by_track
+    trk, hit, out = ctx.upload(w.tracks), ctx.upload(w.hits), ctx.alloc(w.tracks.nbytes)
+    res = {}
+    rows = {"reference fp32 (float oracle)": by_track(w, o32)}
+    for math in ("fast", "fp64gain"):
+        ctx.run(trk, hit, out, nevents, ntrk // 32, 20, 32, math)
+        rows[f"gpu {math}"] = by_track(w, ctx.download(out))
+    for b_ in (trk, hit, out):
+        b_.free()
+    for name, a in rows.items():
+        par, cov, nonfin = per_track_errors(a, o64)
+        res[name] = {"par": quantiles(par), "cov": quantiles(cov), "nonfinite_fp64": nonfin,
+                     "nonfinite_here": int((~np.all(np.isfinite(a[:, :6]), axis=1)).sum())}
+    print("PRECISION_REPORT " + json.dumps({"tracks": int(w.ntracks), "layers": 20, "report": res}), flush=True)
+    f, g = res["gpu fast"], res["gpu fp64gain"]
+    assert g["par"]["median"] <= f["par"]["median"] * 1.01 + 1e-12
+    assert g["cov"]["median"] <= f["cov"]["median"]
+    assert g["cov"]["p99"] <= f["cov"]["p99"]
+    assert g["par"]["frac_le_1e-5"] >= f["par"]["frac_le_1e-5"]

@@ -1,0 +1,151 @@
+"""GPU tests of the serving loop (p2z_server_*): results are the same alone
+and in the mix, admission by the HBM arena, request bookkeeping, and the
+Poisson driver's report.  Failure conditions are forced from the host
+(budgets, ids); every kernel launched runs to completion normally."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import p2z
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg: str) -> None:
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def pool(ctx):
+    """A ragged workload resident on the device."""
+    w = p2z.Workload("ragged", nevents=40, width=32, seed=11)
+    d = dict(w=w, trk=ctx.upload(w.tracks), hit=ctx.upload(w.hits), lanes=ctx.upload(np.ascontiguousarray(w.lane_nlayer)))
+    yield d
+    for k in ("trk", "hit", "lanes"):
+        d[k].free()
+
+
+def event_blocks(w, e):
+    b = np.array(w.blocks[w.event_blocks[e]:w.event_blocks[e + 1]])
+    b["out_block"] -= b["out_block"][0]
+    return b
+
+
+def blk_bytes(W):
+    return 28 * W * 4
+
+
+def test_result_is_the_same_alone_and_in_the_mix(ctx, pool):
+    w = pool["w"]
+    W = w.width
+    budget = (w.nblocks + 16) * blk_bytes(W)
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], budget, W, "fast", nstreams=4) as sv:
+        # mixed: Poisson arrivals fast enough that several events share batches and overlap
+        rep, lat, ob = sv.poisson(w.blocks, w.event_blocks, rate=50000.0, seed=3, release=False)
+        log(f"  mix: {rep.batches} batches, {rep.mean_batch_events:.2f} events/batch, "
+            f"max in flight {rep.max_inflight_events}")
+        nb = np.diff(w.event_blocks)
+        mixed = {}
+        for e in range(w.nevents):
+            c = p2z.Completion(e, int(ob[e]), int(nb[e]), 0, 0, 0, 0)
+            mixed[e] = sv.result(c)
+        for e in range(w.nevents):
+            sv.release(e)
+        # alone: one request at a time
+        for e in range(w.nevents):
+            rid = 1000 + e
+            assert sv.submit(rid, event_blocks(w, e)) == p2z.OK
+            c = sv.wait([rid])[rid]
+            alone = sv.result(c)
+            sv.release(rid)
+            assert np.array_equal(alone.view(np.uint32), mixed[e].view(np.uint32)), e
+    assert rep.batches < w.nevents   # the mix really batched events together
+
+
+def test_matches_the_batch_path(ctx, pool):
+    """A served event equals the same event in one ragged batch launch."""
+    w = pool["w"]
+    W = w.width
+    blk = ctx.upload(np.ascontiguousarray(w.blocks))
+    out = ctx.alloc(w.tracks.nbytes)
+    ctx.run_ragged(pool["trk"], pool["hit"], out, pool["lanes"], blk, w.nblocks, W, "fast")
+    ref = ctx.download(out)
+    blk.free()
+    out.free()
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 64 * 1024 * 1024, W, "fast") as sv:
+        for e in (0, 7, w.nevents - 1):
+            assert sv.submit(e, event_blocks(w, e)) == p2z.OK
+            c = sv.wait([e])[e]
+            b0, b1 = int(w.event_blocks[e]), int(w.event_blocks[e + 1])
+            exp = ref[b0 * 28 * W: b1 * 28 * W]
+            assert np.array_equal(sv.result(c).view(np.uint32), exp.view(np.uint32)), e
+            sv.release(e)
+
+
+def test_admission_is_bounded_by_the_arena(ctx, pool):
+    w = pool["w"]
+    W = w.width
+    nb = np.diff(w.event_blocks)
+    e0, e1 = 0, 1
+    budget = int(max(nb[e0], nb[e1])) * blk_bytes(W)          # room for one of the two at a time
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], budget, W, "fast", nstreams=2) as sv:
+        assert sv.submit(e0, event_blocks(w, e0)) == p2z.OK
+        assert sv.submit(e1, event_blocks(w, e1)) == p2z.EBUSY      # no room until e0 is released
+        sv.wait([e0])
+        assert sv.submit(e1, event_blocks(w, e1)) == p2z.EBUSY      # done but not released: still held
+        sv.release(e0)
+        assert sv.submit(e1, event_blocks(w, e1)) == p2z.OK
+        sv.wait([e1])
+        sv.release(e1)
+    # an event larger than the whole arena is rejected outright
+    small = int(nb.max() - 1) * blk_bytes(W)
+    big = int(np.argmax(nb))
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], small, W, "fast") as sv:
+        with pytest.raises(p2z.P2zError) as err:
+            sv.submit(big, event_blocks(w, big))
+        assert err.value.rc == p2z.EINVAL
+
+
+def test_request_bookkeeping_errors(ctx, pool):
+    w = pool["w"]
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 64 * 1024 * 1024, w.width, "fast") as sv:
+        assert sv.submit(5, event_blocks(w, 5)) == p2z.OK
+        with pytest.raises(p2z.P2zError) as err:
+            sv.submit(5, event_blocks(w, 5))                  # duplicate id
+        assert err.value.rc == p2z.EINVAL
+        with pytest.raises(p2z.P2zError) as err:
+            sv.release(99)                                    # unknown id
+        assert err.value.rc == p2z.EINVAL
+        bad = event_blocks(w, 6)
+        bad["out_block"][0] = 10 ** 6                         # result block outside the request
+        with pytest.raises(p2z.P2zError):
+            sv.submit(6, bad)
+        sv.wait([5])
+        sv.release(5)
+    with pytest.raises(p2z.P2zError):                         # budget below one block
+        p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 100, w.width, "fast")
+
+
+def test_kept_results_overflowing_the_arena_fail_cleanly(ctx, pool):
+    """Poisson run keeping every result in an arena too small for all of them:
+    an error once nothing is in flight, not a wait forever."""
+    w = pool["w"]
+    budget = int(np.diff(w.event_blocks)[:8].sum()) * blk_bytes(w.width)
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], budget, w.width, "fast") as sv:
+        with pytest.raises(p2z.P2zError) as err:
+            sv.poisson(w.blocks, w.event_blocks, rate=20000.0, release=False)
+        assert err.value.rc == p2z.ENOMEM
+
+
+@pytest.mark.parametrize("rate", [2000.0, 20000.0])
+def test_poisson_report(ctx, pool, rate):
+    w = pool["w"]
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 256 * 1024 * 1024, w.width, "fp64gain") as sv:
+        rep, lat, ob = sv.poisson(w.blocks, w.event_blocks, rate=rate, seed=1, release=True)
+    d = rep.as_dict()
+    log(f"  rate {rate}: {d}")
+    assert rep.nevents == w.nevents and rep.batches >= 1
+    assert 0 < rep.lat_p50 <= rep.lat_p99 <= rep.lat_p999 <= rep.lat_max
+    assert np.all(lat > 0)
+    assert abs(np.median(lat) - rep.lat_p50) <= 1e-3 + 0.5 * rep.lat_p50
