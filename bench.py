@@ -135,7 +135,7 @@ def section_ragged(ctx, nevents: int, sample_events: int) -> tuple[dict, dict]:
     import autotune
 
     log(f"ragged: autotuning on {sample_events} events")
-    tuned = autotune.tune(ctx, "ragged", nevents=sample_events, math="fp64gain", reps=3)
+    tuned = autotune.tune(ctx, "ragged", nevents=sample_events, math="fp64gain", reps=3, log=log)
     best = tuned["best"]
     W = best["width"]
     opts = dict(tracks_per_wg=best["tracks_per_wg"], prefetch=best["prefetch"], waves_per_cu=best["waves_per_cu"])
