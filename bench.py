@@ -128,47 +128,72 @@ def time_ms(ctx, fn, reps: int) -> float:
     return ctx.timer_stop() / reps
 
 
+def ragged_bytes(w) -> float:
+    """Algorithmic HBM bytes of one ragged pass: per track 24 B of hit per
+    layer plus its 112-byte state read and written once."""
+    return 24.0 * w.track_layers + 224.0 * w.ntracks
+
+
 def section_ragged(ctx, nevents: int, sample_events: int) -> tuple[dict, dict]:
-    """cfg 3: autotune on a sample, then the full event set through the
-    persistent + hipGraph launch and through one plain launch per event."""
+    """cfg 3: autotune on a sample (per arithmetic mode), then the full event
+    set through the persistent + hipGraph launch, through one one-shot launch
+    over all blocks, and through one plain launch per event."""
     sys.path.insert(0, os.path.join(HERE, "p2z-tests_amd"))
     import autotune
 
-    log(f"ragged: autotuning on {sample_events} events")
-    tuned = autotune.tune(ctx, "ragged", nevents=sample_events, math="fp64gain", reps=3, log=log)
-    best = tuned["best"]
-    W = best["width"]
-    opts = dict(tracks_per_wg=best["tracks_per_wg"], prefetch=best["prefetch"], waves_per_cu=best["waves_per_cu"])
-    log(f"ragged: best {best}; generating {nevents} events at width {W}")
-    w = p2z.Workload("ragged", nevents=nevents, width=W, seed=2024)
-    d = dict(trk=ctx.upload(w.tracks), hit=ctx.upload(w.hits), lanes=ctx.upload(np.ascontiguousarray(w.lane_nlayer)),
-             lpt=ctx.upload(np.ascontiguousarray(w.blocks_lpt)), evt=ctx.upload(np.ascontiguousarray(w.blocks)))
-    out = ctx.alloc(w.tracks.nbytes)
-    math = "fp64gain"
-    res = {"workload": f"{nevents} events, {w.ntracks} tracks, {w.track_layers} track-layers "
-                       f"(tracks/event log-normal median 2000 sigma 1 in [50,20000], layers U{{8..26}})",
-           "math": math, "tuned": {"width": W, **opts, "candidates": tuned["candidates"],
-                                   "sample_events": sample_events}}
-    ms = time_ms(ctx, lambda: ctx.run_ragged(d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, math,
-                                             graph=True, **opts), 5)
-    res["persistent_graph"] = {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3)}
-    ms = time_ms(ctx, lambda: ctx.run_ragged(d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, "fast",
-                                             graph=True, **opts), 5)
-    res["persistent_graph_fp32"] = {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3)}
-    eb = w.event_blocks
+    res = {"math_modes": {}}
+    pools = {}
+    for math in ("fp64gain", "fast"):
+        log(f"ragged: autotuning {math} on {sample_events} events")
+        tuned = autotune.tune(ctx, "ragged", nevents=sample_events, math=math, reps=3, log=log)
+        best = tuned["best"]
+        W = best["width"]
+        opts = dict(tracks_per_wg=best["tracks_per_wg"], prefetch=best["prefetch"], waves_per_cu=best["waves_per_cu"])
+        log(f"ragged: {math} best {best}; generating {nevents} events at width {W}")
+        if W not in pools:
+            w = p2z.Workload("ragged", nevents=nevents, width=W, seed=2024)
+            pools[W] = dict(w=w, d=dict(trk=ctx.upload(w.tracks), hit=ctx.upload(w.hits),
+                                        lanes=ctx.upload(np.ascontiguousarray(w.lane_nlayer)),
+                                        lpt=ctx.upload(np.ascontiguousarray(w.blocks_lpt)),
+                                        evt=ctx.upload(np.ascontiguousarray(w.blocks))))
+        w, d = pools[W]["w"], pools[W]["d"]
+        nbytes = ragged_bytes(w)
+        res["workload"] = (f"{nevents} events, {w.ntracks} tracks, {w.track_layers} track-layers "
+                           f"(tracks/event log-normal median 2000 sigma 1 in [50,20000], layers U{{8..26}})")
+        out = ctx.alloc(w.tracks.nbytes)
+        row = {"tuned": {"width": W, **opts, "candidates": tuned["candidates"], "sample_events": sample_events}}
 
-    def per_event():
-        for e in range(nevents):
-            b0, b1 = int(eb[e]), int(eb[e + 1])
-            ctx.run_ragged(d["trk"], d["hit"], out, d["lanes"], d["evt"].ptr + b0 * 32, b1 - b0, W, math,
-                           persistent=False, tracks_per_wg=opts["tracks_per_wg"])
+        def rate(ms):
+            return {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3),
+                    "hbm_frac": round(nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
 
-    ms = time_ms(ctx, per_event, 1)
-    res["one_launch_per_event"] = {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3),
-                                   "launches": nevents}
-    res["speedup_vs_per_event"] = round(res["persistent_graph"]["tl_per_s"] / res["one_launch_per_event"]["tl_per_s"], 2)
-    out.free()
-    return res, dict(w=w, d=d, W=W, opts=opts)
+        row["persistent_graph"] = rate(time_ms(ctx, lambda: ctx.run_ragged(
+            d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, math, graph=True, **opts), 5))
+        row["oneshot_all_blocks"] = rate(time_ms(ctx, lambda: ctx.run_ragged(
+            d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, math, persistent=False,
+            tracks_per_wg=opts["tracks_per_wg"]), 5))
+        eb = w.event_blocks
+
+        def per_event():
+            for e in range(nevents):
+                b0, b1 = int(eb[e]), int(eb[e + 1])
+                ctx.run_ragged(d["trk"], d["hit"], out, d["lanes"], d["evt"].ptr + b0 * 32, b1 - b0, W, math,
+                               persistent=False, tracks_per_wg=opts["tracks_per_wg"])
+
+        row["one_launch_per_event"] = {**rate(time_ms(ctx, per_event, 1)), "launches": nevents}
+        row["speedup_vs_per_event"] = round(row["persistent_graph"]["tl_per_s"] /
+                                            row["one_launch_per_event"]["tl_per_s"], 2)
+        out.free()
+        res["math_modes"][math] = row
+    res["bytes_per_pass"] = "24 B x track-layers + 224 B x tracks (algorithmic)"
+    W0 = res["math_modes"]["fp64gain"]["tuned"]["width"]
+    opts0 = {k: res["math_modes"]["fp64gain"]["tuned"][k] for k in ("tracks_per_wg", "prefetch", "waves_per_cu")}
+    for W, pool in pools.items():
+        if W != W0:
+            for b in pool["d"].values():
+                b.free()
+            pool["w"].close()
+    return res, dict(w=pools[W0]["w"], d=pools[W0]["d"], W=W0, opts=opts0)
 
 
 def section_serving(ctx, pool: dict, rates, budget_gb: float) -> list:

@@ -156,6 +156,15 @@ def run_fp64(trk, hit, nevts, nb, width, nlayer, nthreads=0) -> np.ndarray:
     return out
 
 
+def fp64_blocks(o64: np.ndarray, trk: np.ndarray, width: int) -> np.ndarray:
+    """Track-major FP64 results (run_fp64) as AoSoA float32 blocks with the
+    input's charges, the layout the report (summary) reads."""
+    nblk = o64.shape[0] // width
+    out = np.ascontiguousarray(o64.reshape(nblk, width, 28).transpose(0, 2, 1)).astype(np.float32)
+    out[:, 27, :] = trk.reshape(nblk, 28, width)[:, 27, :]
+    return out.reshape(-1)
+
+
 def run_ragged(trk, hit, nblocks, width, blk_trk, blk_hit, blk_nlay, lane_nlay, fp64=False, nthreads=0):
     blk_trk = np.ascontiguousarray(blk_trk, np.int64)
     blk_hit = np.ascontiguousarray(blk_hit, np.int64)

@@ -26,7 +26,7 @@ import p2z  # noqa: E402
 WIDTHS = (16, 32, 64, 128)
 TPWS = (64, 128, 256)
 PREFETCH = (1, 2)
-WAVES_PER_CU = (4, 8, 16)
+WAVES_PER_CU = (4, 8, 12, 16, 24, 32)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "p2z", "autotune.json")
 
 

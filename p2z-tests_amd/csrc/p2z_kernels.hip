@@ -46,6 +46,27 @@ __device__ inline Hit load_hit(const float* __restrict__ hb) {
   return h;
 }
 
+// Layers [0, n) of one track whose hits start at hb (consecutive layers 9W
+// words apart), with the next layer's hit in flight while a layer is
+// computed.  Unrolled by two over alternating hit buffers, so the prefetched
+// hit is never copied between registers.  Every load index is < n.
+template <int W, int kMath>
+__device__ inline void run_layers(Track& t, const float* __restrict__ hb, int n) {
+  if (n <= 0) return;
+  Hit a = load_hit<W>(hb);
+  Hit b = a;
+  if (n > 1) b = load_hit<W>(hb + kHitWords * W);
+  layer_step<kMath, true>(t, a);
+  int lay = 1;
+  for (; lay + 1 < n; lay += 2) {  // b holds layer lay
+    a = load_hit<W>(hb + (int64_t)(lay + 1) * (kHitWords * W));
+    layer_step<kMath, false>(t, b);
+    if (lay + 2 < n) b = load_hit<W>(hb + (int64_t)(lay + 2) * (kHitWords * W));
+    layer_step<kMath, false>(t, a);
+  }
+  if (lay < n) layer_step<kMath, false>(t, b);
+}
+
 // ---------------------------------------------------------------------------
 // fused driver pass (reference OMP:733-747 for every block at once)
 // ---------------------------------------------------------------------------
@@ -60,18 +81,7 @@ __global__ __launch_bounds__(256) void k_fused(const float* __restrict__ trk,
   const int l = (int)(g % W);
   Track t;
   load_track<W>(t, trk + b * (kTrkWords * W) + l);
-  const float* hb = hit + b * (int64_t)nlayer * (kHitWords * W) + l;
-  if (nlayer > 0) {
-    Hit h = load_hit<W>(hb);
-    Hit hn = h;
-    if (nlayer > 1) hn = load_hit<W>(hb + kHitWords * W);
-    layer_step<kMath, true>(t, h);
-    for (int lay = 1; lay < nlayer; ++lay) {
-      h = hn;
-      if (lay + 1 < nlayer) hn = load_hit<W>(hb + (int64_t)(lay + 1) * (kHitWords * W));
-      layer_step<kMath, false>(t, h);
-    }
-  }
+  run_layers<W, kMath>(t, hit + b * (int64_t)nlayer * (kHitWords * W) + l, nlayer);
   store_track<W>(t, out + b * (kTrkWords * W) + l);
 }
 
@@ -90,19 +100,17 @@ __device__ inline void ragged_track(const float* __restrict__ trk, const float* 
   int n = lane_nlay[d.trk_block * W + l];
   n = n < d.nlayer ? n : d.nlayer;
   const float* hb = hit + d.hit_block * (kHitWords * W) + l;
-  if (n > 0) {
+  if constexpr (kPrefetch < 2) {
+    run_layers<W, kMath>(t, hb, n);
+  } else if (n > 0) {
     Hit h0 = load_hit<W>(hb), h1 = h0, h2 = h0;
     if (n > 1) h1 = load_hit<W>(hb + kHitWords * W);
-    if (kPrefetch >= 2 && n > 2) h2 = load_hit<W>(hb + 2 * kHitWords * W);
+    if (n > 2) h2 = load_hit<W>(hb + 2 * kHitWords * W);
     layer_step<kMath, true>(t, h0);
     for (int lay = 1; lay < n; ++lay) {
       h0 = h1;
-      if (kPrefetch >= 2) {
-        h1 = h2;
-        if (lay + 2 < n) h2 = load_hit<W>(hb + (int64_t)(lay + 2) * (kHitWords * W));
-      } else if (lay + 1 < n) {
-        h1 = load_hit<W>(hb + (int64_t)(lay + 1) * (kHitWords * W));
-      }
+      h1 = h2;
+      if (lay + 2 < n) h2 = load_hit<W>(hb + (int64_t)(lay + 2) * (kHitWords * W));
       layer_step<kMath, false>(t, h0);
     }
   }
@@ -122,9 +130,12 @@ __device__ inline void ragged_item(const float* trk, const float* hit, float* ou
   ragged_track<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, d, (int)(slot % W));
 }
 
-// Persistent form: each wave takes the next item from a device counter until
-// the items run out (every wave reaches that exit).  The counter starts at 0
-// (k_reset_counter runs first on the same stream).
+// Persistent form: each wave takes items from a device counter until it is
+// handed one past the end (every wave reaches that exit).  A wave claims its
+// next item before computing the current one, so the atomic's round trip
+// overlaps the computation; every item below nitems is claimed exactly once
+// and so computed exactly once.  The counter starts at 0 (k_reset_counter runs
+// first on the same stream).
 template <int W, int kMath, int kPrefetch>
 __global__ __launch_bounds__(256) void k_ragged_persistent(const float* __restrict__ trk,
                                                            const float* __restrict__ hit,
@@ -134,12 +145,14 @@ __global__ __launch_bounds__(256) void k_ragged_persistent(const float* __restri
                                                            int64_t nblocks, unsigned* __restrict__ counter) {
   const int64_t nitems = (nblocks * W + 63) / 64;
   const int lane = threadIdx.x & 63;
-  for (;;) {
-    unsigned mine = 0;
-    if (lane == 0) mine = atomicAdd(counter, 1u);
-    const int64_t item = (int64_t)(unsigned)__shfl((int)mine, 0);
-    if (item >= nitems) break;
+  unsigned mine = 0;
+  if (lane == 0) mine = atomicAdd(counter, 1u);
+  int64_t item = (int64_t)(unsigned)__shfl((int)mine, 0);
+  while (item < nitems) {
+    unsigned next = 0;
+    if (lane == 0) next = atomicAdd(counter, 1u);
     ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, lane);
+    item = (int64_t)(unsigned)__shfl((int)next, 0);
   }
 }
 

@@ -249,23 +249,68 @@ __device__ inline float fast_div(float a, float b) {
   return __builtin_fmaf(__builtin_fmaf(-b, q, a), rb, q);
 }
 
-// The 11 Jacobian entries and the new x, y, z, phi of one track.
+// sin x, cos x and 1 - cos x for |x| <= 1, in FP32 without argument
+// reduction:
+//   sin x = x + x^3 S(x^2),  cos x = 1 + x^2 C(x^2),  1 - cos x = -x^2 C(x^2)
+// with S, C the cubic least-squares fits of gen_sincos_coeffs.py --unit,
+// rounded to float.  Checked on every float in [-1, 1]: sin within 0.83 ulp
+// (0.06% not the nearest float), cos within 1.02 ulp (0.12%), 1 - cos within
+// 2.1 ulp of its own value (no cancellation, unlike 1 - cosf(x)).  About a
+// third of the library sincosf's cost, which also handles range reduction.
+__device__ inline void sincos_unit(float x, float* sp, float* cp, float* omcp) {
+  const float u = x * x;
+  const float t = u * x;
+  float ps = __builtin_fmaf(0x1.6b33ecp-19f, u, -0x1.a00964p-13f);
+  ps = __builtin_fmaf(ps, u, 0x1.111104p-7f);
+  ps = __builtin_fmaf(ps, u, -0x1.555556p-3f);
+  *sp = __builtin_fmaf(t, ps, x);
+  float pc = __builtin_fmaf(0x1.96f3acp-16f, u, -0x1.6bfff6p-10f);
+  pc = __builtin_fmaf(pc, u, 0x1.55553p-5f);
+  pc = __builtin_fmaf(pc, u, -0.5f);
+  *omcp = -(u * pc);
+  *cp = __builtin_fmaf(u, pc, 1.0f);
+}
+
+// sincos_unit where |x| <= 1 (the common case for the turning angle of one
+// layer), the library's reduced-argument sincosf elsewhere.  NaN takes the
+// library branch.
+__device__ inline void sincos_short(float x, float* sp, float* cp, float* omcp) {
+  if (__builtin_fabsf(x) <= 1.0f) {
+    sincos_unit(x, sp, cp, omcp);
+  } else {
+    sincosf(x, sp, cp);
+    *omcp = 1.0f - *cp;
+  }
+}
+
+// The 11 Jacobian entries of one track's step.
 struct Jac {
   float J02, J03, J04, J05, J12, J13, J14, J15, J42, J43, J45;
 };
 
-// Helix step and Jacobian in FP32 with contraction; the expressions keep the
-// reference's shape (OMP:629-666) so that the only differences from it are
-// FMA rounding and the device sin/cos.
+// Helix step and Jacobian (OMP:629-666) in FP32 with contraction, with the
+// Jacobian regrouped on shared products.  With sC = sin(cosP sina),
+// omcC = 1 - cos(cosP sina), B = 1 - sinP sC, C = cosP^2 sC + sinP,
+// E = deltaZ cosa / cosT, G = E sinT pt, kpt = k pt:
+//   J02 = -cosP B sinT cosa / cosT       J12 = cosa sinT (cosP^2 sC - sinP) / cosT
+//   J03 = cosP B G - kpt pt (cosP sina - sinP omcC)
+//   J13 = C G      - kpt pt (sinP sina + cosP omcC)
+//   J04 = kpt (-sinP sina B - cosP omcC) J14 = kpt (cosP sina B - sinP omcC)
+//   J05 = cosP B E / cosT                J15 = C E / cosT
+//   J42 = -ipt sinT icosTk, J43 = sinT deltaZ icosTk, J45 = ipt deltaZ icosT icosTk
+// which is the reference's expressions with the common factors taken out
+// (e.g. -sinP sina + sinP^2 sina sC = -sinP sina (1 - sinP sC)).
 // kAccurateTrig: the track angles' sin/cos through sincos_accurate (the
-// FP64-gain mode); otherwise the library's sincosf, which measures as accurate
-// as the reference at the workload level (error against an FP64 evaluation no
-// larger than the reference's own) at lower cost.
-template <bool kAccurateTrig>
+// FP64-gain mode); otherwise the library's sincosf.  The two small angles
+// (alpha and cosP sina, |cosP sina| <= 1 always) use sincos_short /
+// sincos_unit in both modes, which also give 1 - cos without cancellation.
+// kFirst: J02, J12, J42 are needed (the input z row can be non-zero).
+template <bool kAccurateTrig, bool kFirst = true>
 __device__ inline Jac helix_fast(Track& t, float hz) {
 #pragma clang fp contract(fast)
   float* p = t.p;
   const float k = (float)t.q * kKfact;
+  const float rk = fast_rcp(k);
   const float deltaZ = hz - p[2];
   const float ipt = p[3];
   const float pt = fast_rcp(ipt);
@@ -280,35 +325,48 @@ __device__ inline Jac helix_fast(Track& t, float hz) {
   const float pxin = cosP * pt;
   const float pyin = sinP * pt;
   const float icosT = fast_rcp(cosT);
-  const float icosTk = fast_div(icosT, k);
-  const float alpha = deltaZ * sinT * ipt * icosTk;
-  float sina, cosa;
-  if constexpr (kAccurateTrig)
-    sincos_accurate(alpha, &sina, &cosa);
-  else
-    sincosf(alpha, &sina, &cosa);
-  p[0] = p[0] + k * (pxin * sina - pyin * (1.f - cosa));
-  p[1] = p[1] + k * (pyin * sina + pxin * (1.f - cosa));
+  const float icosTk = icosT * rk;
+  const float dzk = deltaZ * icosTk;
+  const float J43 = sinT * dzk;
+  const float alpha = J43 * ipt;
+  float sina, cosa, omca;
+  sincos_short(alpha, &sina, &cosa, &omca);
+  p[0] = p[0] + k * (pxin * sina - pyin * omca);
+  p[1] = p[1] + k * (pyin * sina + pxin * omca);
   p[2] = hz;
   p[4] = p[4] + alpha;
 
-  float sC, cC;
-  if constexpr (kAccurateTrig)
-    sincos_accurate(cosP * sina, &sC, &cC);
-  else
-    sincosf(cosP * sina, &sC, &cC);
+  const float csa = cosP * sina;
+  float sC, cC, omcC;
+  sincos_unit(csa, &sC, &cC, &omcC);
+  const float ssa = sinP * sina;
+  const float B = __builtin_fmaf(-sinP, sC, 1.0f);
+  const float cc = cosP * cosP;
+  const float C = __builtin_fmaf(cc, sC, sinP);
+  const float E = deltaZ * cosa * icosT;
+  const float F = E * icosT;
+  const float G = E * sinT * pt;
+  const float kpt = k * pt;
+  const float kpp = kpt * pt;
+  const float cO = cosP * omcC, sO = sinP * omcC;
+  const float cB = cosP * B;
   Jac J;
-  J.J02 = cosP * sinT * (sinP * cosa * sC - cosa) * icosT;
-  J.J03 = cosP * sinT * deltaZ * cosa * (1.f - sinP * sC) * (icosT * pt) - k * (cosP * sina - sinP * (1.f - cC)) * (pt * pt);
-  J.J04 = (k * pt) * (-sinP * sina + sinP * sinP * sina * sC - cosP * (1.f - cC));
-  J.J05 = cosP * deltaZ * cosa * (1.f - sinP * sC) * (icosT * icosT);
-  J.J12 = cosa * sinT * (cosP * cosP * sC - sinP) * icosT;
-  J.J13 = sinT * deltaZ * cosa * (cosP * cosP * sC + sinP) * (icosT * pt) - k * (sinP * sina + cosP * (1.f - cC)) * (pt * pt);
-  J.J14 = (k * pt) * (-sinP * (1.f - cC) - sinP * cosP * sina * sC + cosP * sina);
-  J.J15 = deltaZ * cosa * (cosP * cosP * sC + sinP) * (icosT * icosT);
-  J.J42 = -ipt * sinT * icosTk;
-  J.J43 = sinT * deltaZ * icosTk;
-  J.J45 = ipt * deltaZ * (icosT * icosTk);
+  J.J03 = cB * G - kpp * (csa - sO);
+  J.J13 = C * G - kpp * (ssa + cO);
+  J.J04 = kpt * (-ssa * B - cO);
+  J.J14 = kpt * (csa * B - sO);
+  J.J05 = cB * F;
+  J.J15 = C * F;
+  J.J43 = J43;
+  J.J45 = ipt * dzk * icosT;
+  if constexpr (kFirst) {
+    const float sca = sinT * cosa * icosT;
+    J.J02 = -cB * sca;
+    J.J12 = (cc * sC - sinP) * sca;
+    J.J42 = -ipt * sinT * icosTk;
+  } else {
+    J.J02 = J.J12 = J.J42 = 0.f;
+  }
   return J;
 }
 
@@ -380,7 +438,7 @@ __device__ inline void propagate_cov_fast(float* P, const Jac& J) {
 
 template <bool kZRow0, bool kAccurateTrig = false>
 __device__ inline void propagate_fast(Track& t, float hz) {
-  const Jac J = helix_fast<kAccurateTrig>(t, hz);
+  const Jac J = helix_fast<kAccurateTrig, !kZRow0>(t, hz);
   propagate_cov_fast<kZRow0>(t.P, J);
 }
 
@@ -458,8 +516,14 @@ __device__ inline void update_fp64gain(Track& t, const Hit& h) {
     for (int b = 0; b <= a; ++b) Q[a * (a + 1) / 2 + b] = (double)P[S(L::at(a), L::at(b))];
   auto q = [&](int a, int b) -> double { return a >= b ? Q[a * (a + 1) / 2 + b] : Q[b * (b + 1) / 2 + a]; };
   const double R00 = h.xx, R01 = h.xy, R11 = h.yy;
+  // S = H P H^T + R and its inverse: 1/det from the hardware FP64 reciprocal
+  // refined by two Newton steps (each doubles the correct bits; the
+  // estimate has more than 20), i.e. correct to about one ulp of double
   const double r00 = q(0, 0) + R00, r01 = q(1, 0) + R01, r11 = q(1, 1) + R11;
-  const double s = 1.0 / (r00 * r11 - r01 * r01);
+  const double det = r00 * r11 - r01 * r01;
+  double s = __builtin_amdgcn_rcp(det);
+  s = __builtin_fma(s, __builtin_fma(-det, s, 1.0), s);
+  s = __builtin_fma(s, __builtin_fma(-det, s, 1.0), s);
   const double i00 = s * r11, i01 = -s * r01, i11 = s * r00;
   double K0[6], K1[6];
 #pragma unroll
@@ -474,23 +538,22 @@ __device__ inline void update_fp64gain(Track& t, const Hit& h) {
     const int i = L::at(a);
     p[i] = (float)((double)p[i] + K0[a] * res0 + K1[a] * res1);
   }
-  // Joseph form: M = (I - K H) Q ; P' = M (I - K H)^T + K R K^T
-  double M[6][6];
-#pragma unroll
-  for (int a = 0; a < n; ++a)
-#pragma unroll
-    for (int b = 0; b < n; ++b) M[a][b] = q(a, b) - K0[a] * q(0, b) - K1[a] * q(1, b);
-  double KR0[6], KR1[6];
+  // Joseph form (I - K H) P (I - K H)^T + K R K^T, expanded:
+  //   P - K (H P) - (P H^T) K^T + K S K^T,   S = H P H^T + R,
+  // entry (a, b) = P_ab - K_a.(HP)_b + (K_a S - (P H^T)_a).K_b.
+  // D_a = K_a S - (P H^T)_a vanishes for the exact optimal gain; computing it
+  // keeps the form's insensitivity to errors in K.
+  double D0[6], D1[6];
 #pragma unroll
   for (int a = 0; a < n; ++a) {
-    KR0[a] = K0[a] * R00 + K1[a] * R01;
-    KR1[a] = K0[a] * R01 + K1[a] * R11;
+    D0[a] = K0[a] * r00 + K1[a] * r01 - q(a, 0);
+    D1[a] = K0[a] * r01 + K1[a] * r11 - q(a, 1);
   }
 #pragma unroll
   for (int a = 0; a < n; ++a)
 #pragma unroll
     for (int b = 0; b <= a; ++b) {
-      const double v = M[a][b] - M[a][0] * K0[b] - M[a][1] * K1[b] + KR0[a] * K0[b] + KR1[a] * K1[b];
+      const double v = q(a, b) - K0[a] * q(0, b) - K1[a] * q(1, b) + D0[a] * K0[b] + D1[a] * K1[b];
       P[S(L::at(a), L::at(b))] = (float)v;
     }
 }

@@ -45,6 +45,7 @@ def golden():
         layers=npz("ref_track_layers.npz"),
         known_answer=lines("known_answer.txt"),
         ref_summary_full=lines("ref_summary_full.txt"),
+        fp64_summary_full=lines("fp64_summary_full.txt"),
     )
 
 

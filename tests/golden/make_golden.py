@@ -19,6 +19,9 @@ Files:
                         20 layers, after propagateToZ and after KalmanUpdate_v2
   ref_summary_full.txt  the reference's own report at the full configuration
                         (nevts=100 ntrks=9600 nlayer=20), contraction off
+  fp64_summary_full.txt the same report from the FP64 oracle (every
+                        operation of the algorithm in double) on the same
+                        inputs: the answer without FP32 rounding
   ref_steps.npz         8 random blocks (bsize 32, seeded): one propagateToZ and
                         one KalmanUpdate_v2 each, from both reference builds
   ref_spread.json       the reference's own per-track spread on the driver
@@ -158,7 +161,19 @@ def main() -> None:
     assert len(rep) == 11, res.stdout
     with open(os.path.join(HERE, "ref_summary_full.txt"), "w") as f:
         f.write("\n".join(rep) + "\n")
+    write_fp64_summary(t, h)
     print("golden fixtures written to", HERE)
+
+
+def write_fp64_summary(t, h) -> None:
+    nevts, ntrks, W, nl = 100, 9600, 32, 20
+    nb = ntrks // W
+    trk, hit = pyoracle.generate(t, h, nevts, nb, W, nl, 1e-7)
+    o64 = pyoracle.run_fp64(trk, hit, nevts, nb, W, nl)
+    s = pyoracle.summary(pyoracle.fp64_blocks(o64, trk, W), nevts, nb, W, ntrks,
+                         np.asarray(h, np.float32)[nl - 1, :3])
+    with open(os.path.join(HERE, "fp64_summary_full.txt"), "w") as f:
+        f.write(pyoracle.format_summary(s))
 
 
 if __name__ == "__main__":

@@ -210,12 +210,17 @@ def _parse(lines):
     return p(lines)
 
 
-@pytest.mark.parametrize("math", ["ref", "fast"])
+@pytest.mark.parametrize("math", ["ref", "fast", "fp64gain"])
 def test_known_answer_full_size(ctx, golden, math):
     """The reference configuration end to end on the GPU (100 events x 9600
-    tracks x 20 layers): the printed report equals the reference's to the
-    last printed digit (the reference's own builds and hosts differ by one
-    unit there: logs/propagate_gcc_omp.log vs the contraction-off build)."""
+    tracks x 20 layers).
+    REF: the printed report equals the reference's to the last printed digit
+    (the reference's own builds and hosts differ by one unit there:
+    logs/propagate_gcc_omp.log vs the contraction-off build).
+    FAST, FP64GAIN: every printed value is at least as close to the FP64
+    oracle's report as the reference's own value is (the reference's report
+    carries an FP32 rounding bias, e.g. pt 2.0e-4 above the FP64 answer; the
+    production modes may have less of it, not more), to the printed digit."""
     nevts, ntrks, W, nl = 100, 9600, 32, 20
     log(f"case known-answer/{math}")
     trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
@@ -223,9 +228,13 @@ def test_known_answer_full_size(ctx, golden, math):
     s = p2z.summary_host(out, nevts, nb, W, ntrks, last)
     text = p2z.format_summary(s).splitlines()
     log("\n".join(text))
-    for ref_lines in (golden["ref_summary_full"], golden["known_answer"]):
-        for a, b in zip(_parse(text), _parse(ref_lines)):
-            assert abs(a - b) <= 2.01e-6, (a, b)
+    if math == "ref":
+        for ref_lines in (golden["ref_summary_full"], golden["known_answer"]):
+            for a, b in zip(_parse(text), _parse(ref_lines)):
+                assert abs(a - b) <= 2.01e-6, (a, b)
+    else:
+        for a, r, e in zip(_parse(text), _parse(golden["ref_summary_full"]), _parse(golden["fp64_summary_full"])):
+            assert abs(a - e) <= max(abs(r - e), 2.01e-6), (a, r, e)
     assert s.nnans == 0 and s.nfail == 0 and s.ngood == nevts * ntrks
 
 

@@ -119,6 +119,23 @@ def test_known_answer_full_size(golden):
     assert s.nnans == 0 and s.nfail == 0 and s.ngood == nevts * ntrks
 
 
+def test_fp64_known_answer_full_size(golden):
+    """The FP64 oracle's report at the full reference configuration is the
+    committed fixture (the answer without FP32 rounding).  The reference's
+    FP32 report departs from it by its own rounding bias: the tracks are
+    copies of one track smeared by 1e-7, so their rounding errors do not
+    average out (pt: 12.328097 against 12.327897)."""
+    t, h = ref_inputs(golden)
+    nevts, ntrks, W, nl = 100, 9600, 32, 20
+    nb = ntrks // W
+    trk, hit = O.generate(t, h, nevts, nb, W, nl, 1e-7)
+    o64 = O.run_fp64(trk, hit, nevts, nb, W, nl)
+    s = O.summary(O.fp64_blocks(o64, trk, W), nevts, nb, W, ntrks, h[nl - 1, :3].astype(np.float32))
+    assert O.format_summary(s).splitlines() == golden["fp64_summary_full"]
+    bias = [abs(a - b) for a, b in zip(_parse(golden["ref_summary_full"]), _parse(golden["fp64_summary_full"]))]
+    assert max(bias) < 1e-3
+
+
 # ---------------------------------------------------------------------------
 # mathematics
 # ---------------------------------------------------------------------------
