@@ -117,6 +117,21 @@ int p2z_propagate_to_z(p2z_ctx* ctx, const float* inErr, const float* inPar, con
 int p2z_kalman_update(p2z_ctx* ctx, float* trkErr, float* inPar, const float* hitErr,
                       const float* msP, int64_t nblocks, int64_t trk_stride, int64_t hit_stride,
                       int32_t width, int32_t math);
+/* KalmanUpdate v1 (OMP:375-501: 3-D hit, 3x3 inverse), in place, same stride
+ * convention; hitErr points at the 6 hit covariance words of each lane.  The
+ * reference does not call it (commented out at OMP:743; the SYCL variant does,
+ * src/propagate-toz-test_sycl.cpp:718).
+ *   flags 0: as written -- it addresses both covariances as upper triangles
+ *     stored row by row (the rest of the path stores lower triangles), its
+ *     inverse entry (0,2) is m01*m12 - m02*m12 (OMP:395), and it writes
+ *     pt + K*res into the ipt slot (OMP:447, 494); reference typing.
+ *   P2Z_V1_FIXED: the textbook 3-D update in the path's packing (lower
+ *     triangles, correct inverse in double, residual in x, y, z, ipt updated
+ *     as ipt). */
+#define P2Z_V1_FIXED 1u
+int p2z_kalman_update_v1(p2z_ctx* ctx, float* trkErr, float* inPar, const float* hitErr,
+                         const float* msP, int64_t nblocks, int64_t trk_stride, int64_t hit_stride,
+                         int32_t width, uint32_t flags);
 /* The driver loop (OMP:731-748) for one pass: out = trk, then nlayer x
  * (propagateToZ, KalmanUpdate_v2) per block, fused in one launch.  trk, hit,
  * out are device pointers (MPTRK[nevts*nb], MPHIT[nlayer*nevts*nb],

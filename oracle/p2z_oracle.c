@@ -352,6 +352,125 @@ void oracle_kalman_update_v2_fp64(float* trkErr, float* inPar, const float* hitE
   }
 }
 
+/* ---- KalmanUpdate v1 (OMP:375-501), off the reference's path ---------------
+ * A 3-D hit update with a 3x3 inverse, commented out at its call site
+ * (OMP:743); the SYCL variant calls it (src/propagate-toz-test_sycl.cpp:718).
+ *
+ * fixed == 0 restates it as written, including three properties of that code
+ * that a caller has to know about:
+ *  - it addresses the 21 covariance words as the UPPER triangle row by row
+ *    ((0,0) (0,1) .. (0,5) (1,1) ..), and the 6 hit covariance words likewise
+ *    (xx xy xz yy yz zz), while propagateToZ and KalmanUpdate_v2 store the
+ *    LOWER triangle row by row (OMP:52-55);
+ *  - entry (0,2) of the inverse multiplies m01*m12 - m02*m12 (OMP:395), where
+ *    the cofactor is m01*m12 - m02*m11 (entry (2,0), OMP:399, has it right);
+ *  - it writes pt + K*res (pt = 1/ipt) into the ipt slot (OMP:447, 494).
+ * Typing as written: the sums and the determinant expression in float, the
+ * determinant widened to double, 1.0/det and the products with it in double,
+ * pt = 1./ipt in double; no contraction.
+ *
+ * fixed != 0 is the update the code means, in the path's own packing: lower
+ * triangles for the track and the hit covariance, residual (x, y, z) -
+ * H p with H = [I3 0], S = H P H^T + R inverted by cofactors with the
+ * determinant and the inverse in double, K = P H^T S^-1, p' = p + K res (ipt
+ * updated as ipt, the state's own parameter), P' = P - K H P; float
+ * elsewhere, no contraction. */
+static int lo_idx(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+static int up_idx(int i, int j) {
+  if (i > j) { const int t = i; i = j; j = t; }
+  return i * 6 - i * (i - 1) / 2 + (j - i);
+}
+
+static void lane_update_v1_ref(float a[21], float par[6], const float b[6], float xout, float yout) {
+  const float m00 = a[0] + b[0], m01 = a[1] + b[1], m02 = a[2] + b[2];
+  const float m11 = a[6] + b[3], m12 = a[7] + b[4], m22 = a[11] + b[5];
+  const double det = (m00 * ((m11 * m22) - (m12 * m12))) - (m01 * ((m01 * m22) - (m12 * m02))) +
+                     (m02 * ((m01 * m12) - (m02 * m11)));
+  const double invdet = 1.0 / det;
+  float c[9];
+  c[0] = invdet * ((m11 * m22) - (m12 * m12));
+  c[1] = -1 * invdet * ((m01 * m22) - (m02 * m12));
+  c[2] = invdet * ((m01 * m12) - (m02 * m12));
+  c[3] = -1 * invdet * ((m01 * m22) - (m12 * m02));
+  c[4] = invdet * ((m00 * m22) - (m02 * m02));
+  c[5] = -1 * invdet * ((m00 * m12) - (m02 * m01));
+  c[6] = invdet * ((m01 * m12) - (m02 * m11));
+  c[7] = -1 * invdet * ((m00 * m12) - (m02 * m01));
+  c[8] = invdet * ((m00 * m11) - (m01 * m01));
+  /* K (OMP:418-436): row i of K uses the words of P(i, 0..2) in the upper
+   * addressing: i = 0: 0 1 2, 1: 1 6 7, 2: 2 7 11, 3: 3 8 12, 4: 4 9 13, 5: 5 10 14 */
+  static const int row[6][3] = {{0, 1, 2}, {1, 6, 7}, {2, 7, 11}, {3, 8, 12}, {4, 9, 13}, {5, 10, 14}};
+  float K[18];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 3; ++j)
+      K[3 * i + j] = a[row[i][0]] * c[j] + a[row[i][1]] * c[3 + j] + a[row[i][2]] * c[6 + j];
+  const float xin = par[0], yin = par[1], zin = par[2];
+  const float ptin = 1. / par[3];
+  const float phiin = par[4], thetain = par[5];
+  const float rx = xout - xin, ry = yout - yin;
+  const float xnew = xin + (K[0] * rx) + (K[1] * ry);
+  const float ynew = yin + (K[3] * rx) + (K[4] * ry);
+  const float znew = zin + (K[6] * rx) + (K[7] * ry);
+  const float ptnew = ptin + (K[9] * rx) + (K[10] * ry);
+  const float phinew = phiin + (K[12] * rx) + (K[13] * ry);
+  const float thetanew = thetain + (K[15] * rx) + (K[16] * ry);
+  /* P' (OMP:451-486): word (i, j) of the upper addressing minus
+   * sum_m K(i, m) * word (m, j) */
+  float n[21];
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) {
+      const int e = up_idx(i, j);
+      n[e] = a[e] - (K[3 * i] * a[up_idx(0, j)] + K[3 * i + 1] * a[up_idx(1, j)] + K[3 * i + 2] * a[up_idx(2, j)]);
+    }
+  for (int e = 0; e < 21; ++e) a[e] = n[e];
+  par[0] = xnew; par[1] = ynew; par[2] = znew;
+  par[3] = ptnew; par[4] = phinew; par[5] = thetanew;
+}
+
+static void lane_update_v1_fixed(float P[21], float par[6], const float b[6], const float hit[3]) {
+  /* S = H P H^T + R, lower packing of both */
+  float m[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) m[i][j] = P[lo_idx(i, j)] + b[lo_idx(i, j)];
+  const double m00 = m[0][0], m01 = m[1][0], m02 = m[2][0], m11 = m[1][1], m12 = m[2][1], m22 = m[2][2];
+  const double c00 = m11 * m22 - m12 * m12, c01 = m02 * m12 - m01 * m22, c02 = m01 * m12 - m02 * m11;
+  const double c11 = m00 * m22 - m02 * m02, c12 = m01 * m02 - m00 * m12, c22 = m00 * m11 - m01 * m01;
+  const double det = m00 * c00 + m01 * c01 + m02 * c02;
+  const double id = 1.0 / det;
+  const float inv[3][3] = {{(float)(c00 * id), (float)(c01 * id), (float)(c02 * id)},
+                           {(float)(c01 * id), (float)(c11 * id), (float)(c12 * id)},
+                           {(float)(c02 * id), (float)(c12 * id), (float)(c22 * id)}};
+  float K[6][3];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 3; ++j)
+      K[i][j] = P[lo_idx(i, 0)] * inv[0][j] + P[lo_idx(i, 1)] * inv[1][j] + P[lo_idx(i, 2)] * inv[2][j];
+  const float r[3] = {hit[0] - par[0], hit[1] - par[1], hit[2] - par[2]};
+  float n[21];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j <= i; ++j)
+      n[lo_idx(i, j)] = P[lo_idx(i, j)] -
+                        (K[i][0] * P[lo_idx(0, j)] + K[i][1] * P[lo_idx(1, j)] + K[i][2] * P[lo_idx(2, j)]);
+  for (int i = 0; i < 6; ++i) par[i] = par[i] + K[i][0] * r[0] + K[i][1] * r[1] + K[i][2] * r[2];
+  for (int e = 0; e < 21; ++e) P[e] = n[e];
+}
+
+void oracle_kalman_update_v1(float* trkErr, float* inPar, const float* hitErr, const float* msP, int W,
+                             int fixed) {
+  for (int it = 0; it < W; ++it) {
+    float P[21], par[6], hc[6];
+    for (int e = 0; e < 21; ++e) P[e] = trkErr[e * W + it];
+    for (int e = 0; e < 6; ++e) par[e] = inPar[e * W + it];
+    for (int e = 0; e < 6; ++e) hc[e] = hitErr[e * W + it];
+    const float hit[3] = {msP[0 * W + it], msP[1 * W + it], msP[2 * W + it]};
+    if (fixed)
+      lane_update_v1_fixed(P, par, hc, hit);
+    else
+      lane_update_v1_ref(P, par, hc, hit[0], hit[1]);
+    for (int e = 0; e < 21; ++e) trkErr[e * W + it] = P[e];
+    for (int e = 0; e < 6; ++e) inPar[e * W + it] = par[e];
+  }
+}
+
 /* One block through nlayer layers (body of OMP:736-746). */
 static void run_block(const float* tb, const float* hb0, float* ob, int W, int nlayer) {
   memcpy(ob, tb, sizeof(float) * ORACLE_TRK_WORDS * W);

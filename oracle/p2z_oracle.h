@@ -51,6 +51,13 @@ void oracle_propagate_to_z(const float* inErr, const float* inPar, const int32_t
 void oracle_kalman_update_v2(float* trkErr, float* inPar, const float* hitErr,
                              const float* msP, int W);
 
+/* KalmanUpdate v1 (OMP:375-501), in place; hitErr has the 6 hit covariance
+ * words per lane.  fixed == 0: as written (upper-triangle addressing of both
+ * covariances, the (0,2) cofactor of OMP:395, pt written to the ipt slot);
+ * fixed != 0: the textbook 3-D update in the path's lower-triangle packing. */
+void oracle_kalman_update_v1(float* trkErr, float* inPar, const float* hitErr,
+                             const float* msP, int W, int fixed);
+
 /* Driver (OMP:731-748): out = trk, then nlayer x (propagateToZ, KalmanUpdate_v2)
  * per block.  Parallel over (event x block) with nthreads OpenMP threads
  * (<=0: runtime default).  Hits may be per-track ragged via nlay_lane (NULL =

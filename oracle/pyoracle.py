@@ -50,6 +50,7 @@ _SIGS = {
     "oracle_prepare_hits": ([C.POINTER(OAHIT), C.c_int, vp, i64, i64, C.c_int, C.c_int, dbl], C.c_int),
     "oracle_propagate_to_z": ([vp, vp, vp, vp, vp, vp, C.c_int], None),
     "oracle_kalman_update_v2": ([vp, vp, vp, vp, C.c_int], None),
+    "oracle_kalman_update_v1": ([vp, vp, vp, vp, C.c_int, C.c_int], None),
     "oracle_propagate_to_z_fp64": ([vp, vp, vp, vp, vp, vp, C.c_int], None),
     "oracle_kalman_update_v2_fp64": ([vp, vp, vp, vp, C.c_int], None),
     "oracle_run": ([vp, vp, vp, i64, i64, C.c_int, C.c_int, C.c_int], None),
@@ -194,6 +195,16 @@ def update_block(trk_blk, hit_blk, width) -> np.ndarray:
     return out
 
 
+def update_v1_block(trk_blk, hit_blk, width, fixed: bool) -> np.ndarray:
+    """oracle KalmanUpdate v1 (OMP:375-501) on one MPTRK block; fixed selects
+    the textbook form in the path's packing (returns a new block)."""
+    W = width
+    out = trk_blk.copy()
+    lib().oracle_kalman_update_v1(_p(out) + 6 * W * 4, _p(out), _p(hit_blk) + 3 * W * 4, _p(hit_blk), W,
+                                  int(bool(fixed)))
+    return out
+
+
 def propagate_block_fp64(trk_blk, hit_blk, width) -> np.ndarray:
     """propagateToZ on one block in double, rounded to float."""
     W = width
@@ -243,6 +254,7 @@ def ref_lib(name: str):
     L.ref_inputs.restype = C.c_int
     L.ref_propagate_block.argtypes = [vp, vp, vp]
     L.ref_update_block.argtypes = [vp, vp]
+    L.ref_update_v1_block.argtypes = [vp, vp]
     L.ref_randn.argtypes = [C.c_float, C.c_float]
     L.ref_randn.restype = C.c_float
     L.ref_srand.argtypes = [C.c_uint]

@@ -97,6 +97,13 @@ void ref_update_block(float* trk_inout, const float* hit_in) {
   KalmanUpdate_v2(&t->cov, &t->par, &h->cov, &h->pos);
 }
 
+// KalmanUpdate v1 (OMP:375-501, not called by the reference's driver).
+void ref_update_v1_block(float* trk_inout, const float* hit_in) {
+  MPTRK* t = reinterpret_cast<MPTRK*>(trk_inout);
+  const MPHIT* h = reinterpret_cast<const MPHIT*>(hit_in);
+  KalmanUpdate(&t->cov, &t->par, &h->cov, &h->pos);
+}
+
 // The reference program itself (prints the full report to stdout).
 int ref_main() { return p2z_reference_main(0, nullptr); }
 

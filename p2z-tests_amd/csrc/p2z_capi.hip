@@ -261,6 +261,22 @@ int p2z_kalman_update(p2z_ctx* c, float* trkErr, float* inPar, const float* hitE
   return P2Z_OK;
 }
 
+int p2z_kalman_update_v1(p2z_ctx* c, float* trkErr, float* inPar, const float* hitErr, const float* msP,
+                         int64_t nblocks, int64_t trk_stride, int64_t hit_stride, int32_t width, uint32_t flags) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  if (!trkErr || !inPar || !hitErr || !msP) return p2z::fail(P2Z_EINVAL, "null buffer");
+  if (!p2z::valid_width(width)) return p2z::fail(P2Z_EINVAL, "width must be 16/32/64/128 (got %d)", width);
+  if (flags & ~P2Z_V1_FIXED) return p2z::fail(P2Z_EINVAL, "unknown v1 flags 0x%x", flags);
+  if (nblocks < 0 || nblocks * (int64_t)width > (int64_t)1 << 31)
+    return p2z::fail(P2Z_EINVAL, "bad block count %lld", (long long)nblocks);
+  if (trk_stride < 0 || hit_stride < 0)
+    return p2z::fail(P2Z_EINVAL, "bad strides (trk %lld, hit %lld)", (long long)trk_stride, (long long)hit_stride);
+  P2Z_TRY(use_device(c));
+  P2Z_HIP_CHECK(p2z::launch_update_v1(trkErr, inPar, hitErr, msP, nblocks, trk_stride, hit_stride, width,
+                                      (flags & P2Z_V1_FIXED) != 0, c->cfg, c->stream));
+  return P2Z_OK;
+}
+
 int p2z_run(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_t nevts, int64_t nb,
             int32_t nlayer, int32_t width, int32_t math) {
   if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");

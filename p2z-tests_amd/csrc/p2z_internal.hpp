@@ -31,6 +31,10 @@ hipError_t launch_update(float* trkErr, float* inPar, const float* hitErr, const
                          int64_t nblocks, int64_t ts, int64_t hs, int width, int math,
                          const LaunchCfg& cfg, hipStream_t s);
 
+// KalmanUpdate v1 on blocks (fixed: the textbook form).
+hipError_t launch_update_v1(float* trkErr, float* inPar, const float* hitErr, const float* msP, int64_t nblocks,
+                            int64_t ts, int64_t hs, int width, bool fixed, const LaunchCfg& cfg, hipStream_t s);
+
 // Block descriptor of the ragged format (same layout as p2z_block_desc).
 struct BlockDesc {
   int64_t trk_block;

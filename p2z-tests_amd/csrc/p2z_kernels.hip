@@ -53,18 +53,19 @@ __device__ inline Hit load_hit(const float* __restrict__ hb) {
 template <int W, int kMath>
 __device__ inline void run_layers(Track& t, const float* __restrict__ hb, int n) {
   if (n <= 0) return;
+  Trig g;
   Hit a = load_hit<W>(hb);
   Hit b = a;
   if (n > 1) b = load_hit<W>(hb + kHitWords * W);
-  layer_step<kMath, true>(t, a);
+  layer_step<kMath, true>(t, a, g);
   int lay = 1;
   for (; lay + 1 < n; lay += 2) {  // b holds layer lay
     a = load_hit<W>(hb + (int64_t)(lay + 1) * (kHitWords * W));
-    layer_step<kMath, false>(t, b);
+    layer_step<kMath, false>(t, b, g);
     if (lay + 2 < n) b = load_hit<W>(hb + (int64_t)(lay + 2) * (kHitWords * W));
-    layer_step<kMath, false>(t, a);
+    layer_step<kMath, false>(t, a, g);
   }
-  if (lay < n) layer_step<kMath, false>(t, b);
+  if (lay < n) layer_step<kMath, false>(t, b, g);
 }
 
 // ---------------------------------------------------------------------------
@@ -103,15 +104,16 @@ __device__ inline void ragged_track(const float* __restrict__ trk, const float* 
   if constexpr (kPrefetch < 2) {
     run_layers<W, kMath>(t, hb, n);
   } else if (n > 0) {
+    Trig g;
     Hit h0 = load_hit<W>(hb), h1 = h0, h2 = h0;
     if (n > 1) h1 = load_hit<W>(hb + kHitWords * W);
     if (n > 2) h2 = load_hit<W>(hb + 2 * kHitWords * W);
-    layer_step<kMath, true>(t, h0);
+    layer_step<kMath, true>(t, h0, g);
     for (int lay = 1; lay < n; ++lay) {
       h0 = h1;
       h1 = h2;
       if (lay + 2 < n) h2 = load_hit<W>(hb + (int64_t)(lay + 2) * (kHitWords * W));
-      layer_step<kMath, false>(t, h0);
+      layer_step<kMath, false>(t, h0, g);
     }
   }
   store_track<W>(t, out + d.out_block * (kTrkWords * W) + l);
@@ -233,6 +235,37 @@ __global__ __launch_bounds__(256) void k_update(float* trkErr, float* inPar, con
     update_fast<false>(t, h);
   else
     update_fp64gain<false>(t, h);
+#pragma unroll
+  for (int e = 0; e < 21; ++e) trkErr[o + e * W] = t.P[e];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) inPar[o + e * W] = t.p[e];
+}
+
+// KalmanUpdate v1 (OMP:375-501) on blocks, in place; kFixed selects the
+// textbook form.  hitErr: the 6 hit covariance words of each lane.
+template <int W, bool kFixed>
+__global__ __launch_bounds__(256) void k_update_v1(float* trkErr, float* inPar, const float* hitErr,
+                                                   const float* msP, int64_t ntracks, int64_t ts,
+                                                   int64_t hs) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntracks) return;
+  const int64_t b = g / W;
+  const int l = (int)(g % W);
+  Track t;
+  const int64_t o = b * ts + l;
+#pragma unroll
+  for (int e = 0; e < 21; ++e) t.P[e] = trkErr[o + e * W];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) t.p[e] = inPar[o + e * W];
+  t.q = 0;
+  const int64_t ho = b * hs + l;
+  float hc[6];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) hc[e] = hitErr[ho + e * W];
+  if constexpr (kFixed)
+    update_v1_fixed(t, hc, msP[ho], msP[ho + W], msP[ho + 2 * W]);
+  else
+    update_v1_ref(t, hc, msP[ho], msP[ho + W]);
 #pragma unroll
   for (int e = 0; e < 21; ++e) trkErr[o + e * W] = t.P[e];
 #pragma unroll
@@ -434,6 +467,23 @@ hipError_t launch_update(float* trkErr, float* inPar, const float* hitErr, const
                          trkErr, inPar, hitErr, msP, ntracks, ts, hs);
       return hipGetLastError();
     });
+  });
+}
+
+hipError_t launch_update_v1(float* trkErr, float* inPar, const float* hitErr, const float* msP, int64_t nblocks,
+                            int64_t ts, int64_t hs, int width, bool fixed, const LaunchCfg& cfg, hipStream_t s) {
+  const int64_t ntracks = nblocks * width;
+  if (ntracks == 0) return hipSuccess;
+  const int tpw = cfg.tracks_per_wg;
+  return dispatch_width(width, [&](auto wc) {
+    constexpr int W = decltype(wc)::value;
+    if (fixed)
+      hipLaunchKernelGGL((k_update_v1<W, true>), dim3((unsigned)grid_for(ntracks, tpw)), dim3(tpw), 0, s, trkErr,
+                         inPar, hitErr, msP, ntracks, ts, hs);
+    else
+      hipLaunchKernelGGL((k_update_v1<W, false>), dim3((unsigned)grid_for(ntracks, tpw)), dim3(tpw), 0, s, trkErr,
+                         inPar, hitErr, msP, ntracks, ts, hs);
+    return hipGetLastError();
   });
 }
 

@@ -288,6 +288,41 @@ struct Jac {
   float J02, J03, J04, J05, J12, J13, J14, J15, J42, J43, J45;
 };
 
+// sin and cos of a track's phi and theta.  The fused layer loops carry them
+// from layer to layer, rotating them by each change of the angles (alpha in
+// the propagation, the update's corrections) instead of evaluating sin/cos of
+// the updated float angles again: the rotation tracks the exact sum of the
+// increments, where sin of the rounded angle carries that angle's rounding
+// (measured over 40 steps: median 1.3 ulp either way, p99 4.8 ulp rotating
+// against 34 ulp recomputing, the latter near the zeros of sin).
+struct Trig {
+  float sinP, cosP, sinT, cosT;
+};
+
+// kAccurateTrig: sincos_accurate (correctly rounded but for rare cases);
+// otherwise the library's sincosf.
+template <bool kAccurateTrig>
+__device__ inline void trig_init(const Track& t, Trig& g) {
+  if constexpr (kAccurateTrig) {
+    sincos_accurate(t.p[4], &g.sinP, &g.cosP);
+    sincos_accurate(t.p[5], &g.sinT, &g.cosT);
+  } else {
+    sincosf(t.p[4], &g.sinP, &g.cosP);
+    sincosf(t.p[5], &g.sinT, &g.cosT);
+  }
+}
+
+// (s, c) = (sin x, cos x) -> (sin, cos)(x + d), given sin d and 1 - cos d:
+//   sin(x + d) = s + (c sin d - s (1 - cos d)),  cos(x + d) = c - (s sin d + c (1 - cos d))
+// (small corrections added to the values, no product by cos d).
+__device__ inline void rotate(float& s, float& c, float sd, float omcd) {
+#pragma clang fp contract(fast)
+  const float hs = c * sd - s * omcd;
+  const float hc = s * sd + c * omcd;
+  s = s + hs;
+  c = c - hc;
+}
+
 // Helix step and Jacobian (OMP:629-666) in FP32 with contraction, with the
 // Jacobian regrouped on shared products.  With sC = sin(cosP sina),
 // omcC = 1 - cos(cosP sina), B = 1 - sinP sC, C = cosP^2 sC + sinP,
@@ -299,31 +334,23 @@ struct Jac {
 //   J05 = cosP B E / cosT                J15 = C E / cosT
 //   J42 = -ipt sinT icosTk, J43 = sinT deltaZ icosTk, J45 = ipt deltaZ icosT icosTk
 // which is the reference's expressions with the common factors taken out
-// (e.g. -sinP sina + sinP^2 sina sC = -sinP sina (1 - sinP sC)).
-// kAccurateTrig: the track angles' sin/cos through sincos_accurate (the
-// FP64-gain mode); otherwise the library's sincosf.  The two small angles
-// (alpha and cosP sina, |cosP sina| <= 1 always) use sincos_short /
-// sincos_unit in both modes, which also give 1 - cos without cancellation.
+// (e.g. -sinP sina + sinP^2 sina sC = -sinP sina (1 - sinP sC)); likewise
+//   x' = x + kpt (cosP sina - sinP (1 - cosa)),  y' = y + kpt (sinP sina + cosP (1 - cosa)),
+// whose brackets are also the changes of sin phi and cos phi under phi' = phi + alpha
+// (g is rotated with them).  The two small angles (alpha and cosP sina,
+// |cosP sina| <= 1 always) use sincos_short / sincos_unit, which give 1 - cos
+// without cancellation.
 // kFirst: J02, J12, J42 are needed (the input z row can be non-zero).
-template <bool kAccurateTrig, bool kFirst = true>
-__device__ inline Jac helix_fast(Track& t, float hz) {
+template <bool kFirst>
+__device__ inline Jac helix_core(Track& t, float hz, Trig& g) {
 #pragma clang fp contract(fast)
   float* p = t.p;
+  const float sinP = g.sinP, cosP = g.cosP, sinT = g.sinT, cosT = g.cosT;
   const float k = (float)t.q * kKfact;
   const float rk = fast_rcp(k);
   const float deltaZ = hz - p[2];
   const float ipt = p[3];
   const float pt = fast_rcp(ipt);
-  float sinP, cosP, sinT, cosT;
-  if constexpr (kAccurateTrig) {
-    sincos_accurate(p[4], &sinP, &cosP);
-    sincos_accurate(p[5], &sinT, &cosT);
-  } else {
-    sincosf(p[4], &sinP, &cosP);
-    sincosf(p[5], &sinT, &cosT);
-  }
-  const float pxin = cosP * pt;
-  const float pyin = sinP * pt;
   const float icosT = fast_rcp(cosT);
   const float icosTk = icosT * rk;
   const float dzk = deltaZ * icosTk;
@@ -331,22 +358,26 @@ __device__ inline Jac helix_fast(Track& t, float hz) {
   const float alpha = J43 * ipt;
   float sina, cosa, omca;
   sincos_short(alpha, &sina, &cosa, &omca);
-  p[0] = p[0] + k * (pxin * sina - pyin * omca);
-  p[1] = p[1] + k * (pyin * sina + pxin * omca);
+  const float csa = cosP * sina;
+  const float ssa = sinP * sina;
+  const float kpt = k * pt;
+  const float hx = csa - sinP * omca;
+  const float hy = ssa + cosP * omca;
+  p[0] = p[0] + kpt * hx;
+  p[1] = p[1] + kpt * hy;
   p[2] = hz;
   p[4] = p[4] + alpha;
+  g.sinP = sinP + hx;
+  g.cosP = cosP - hy;
 
-  const float csa = cosP * sina;
   float sC, cC, omcC;
   sincos_unit(csa, &sC, &cC, &omcC);
-  const float ssa = sinP * sina;
   const float B = __builtin_fmaf(-sinP, sC, 1.0f);
   const float cc = cosP * cosP;
   const float C = __builtin_fmaf(cc, sC, sinP);
   const float E = deltaZ * cosa * icosT;
   const float F = E * icosT;
   const float G = E * sinT * pt;
-  const float kpt = k * pt;
   const float kpp = kpt * pt;
   const float cO = cosP * omcC, sO = sinP * omcC;
   const float cB = cosP * B;
@@ -368,6 +399,15 @@ __device__ inline Jac helix_fast(Track& t, float hz) {
     J.J02 = J.J12 = J.J42 = 0.f;
   }
   return J;
+}
+
+// One step from the track's own angles (the per-function entry points).
+// kAccurateTrig: sin/cos of phi and theta through sincos_accurate.
+template <bool kAccurateTrig, bool kFirst = true>
+__device__ inline Jac helix_fast(Track& t, float hz) {
+  Trig g;
+  trig_init<kAccurateTrig>(t, g);
+  return helix_core<kFirst>(t, hz, g);
 }
 
 // P' = Jt P Jt^T.  kZRow0: the input z row/column is known to be zero.
@@ -451,8 +491,10 @@ struct Live {
   __device__ static constexpr int at(int a) { return kZRow0 ? (a < 2 ? a : a + 1) : a; }
 };
 
-template <bool kZRow0>
-__device__ inline void update_fast(Track& t, const Hit& h) {
+// kCarry: g holds sin/cos of phi and theta, rotated here by the update's
+// corrections of the two angles.
+template <bool kZRow0, bool kCarry = false>
+__device__ inline void update_fast(Track& t, const Hit& h, Trig* g = nullptr) {
 #pragma clang fp contract(fast)
   using L = Live<kZRow0>;
   float* p = t.p;
@@ -478,7 +520,19 @@ __device__ inline void update_fast(Track& t, const Hit& h) {
 #pragma unroll
   for (int n = 0; n < L::n; ++n) {
     const int i = L::at(n);
+    if (kCarry && i >= 4) continue;
     p[i] = p[i] + K0[i] * res0 + K1[i] * res1;
+  }
+  if constexpr (kCarry) {
+    const float dphi = K0[4] * res0 + K1[4] * res1;
+    const float dtheta = K0[5] * res0 + K1[5] * res1;
+    p[4] = p[4] + dphi;
+    p[5] = p[5] + dtheta;
+    float sd, cd, omcd;
+    sincos_short(dphi, &sd, &cd, &omcd);
+    rotate(g->sinP, g->cosP, sd, omcd);
+    sincos_short(dtheta, &sd, &cd, &omcd);
+    rotate(g->sinT, g->cosT, sd, omcd);
   }
   // P - K (H P); H P = rows 0 and 1 of P.
   float HP0[6], HP1[6];
@@ -501,8 +555,10 @@ __device__ inline void update_fast(Track& t, const Hit& h) {
 // kFp64Gain: FP64 inverse, gain, parameter update and Joseph-form covariance
 // ---------------------------------------------------------------------------
 
-template <bool kZRow0>
-__device__ inline void update_fp64gain(Track& t, const Hit& h) {
+// kCarry: g holds sin/cos of phi and theta, rotated here by the update's
+// corrections of the two angles.
+template <bool kZRow0, bool kCarry = false>
+__device__ inline void update_fp64gain(Track& t, const Hit& h, Trig* g = nullptr) {
 #pragma clang fp contract(fast)
   using L = Live<kZRow0>;
   constexpr int n = L::n;
@@ -533,10 +589,20 @@ __device__ inline void update_fp64gain(Track& t, const Hit& h) {
   }
   const double res0 = (double)h.x - (double)p[0];
   const double res1 = (double)h.y - (double)p[1];
+  float dang[2] = {0.f, 0.f};
 #pragma unroll
   for (int a = 0; a < n; ++a) {
     const int i = L::at(a);
-    p[i] = (float)((double)p[i] + K0[a] * res0 + K1[a] * res1);
+    const double d = K0[a] * res0 + K1[a] * res1;
+    p[i] = (float)((double)p[i] + d);
+    if (i >= 4) dang[i - 4] = (float)d;
+  }
+  if constexpr (kCarry) {
+    float sd, cd, omcd;
+    sincos_short(dang[0], &sd, &cd, &omcd);
+    rotate(g->sinP, g->cosP, sd, omcd);
+    sincos_short(dang[1], &sd, &cd, &omcd);
+    rotate(g->sinT, g->cosT, sd, omcd);
   }
   // Joseph form (I - K H) P (I - K H)^T + K R K^T, expanded:
   //   P - K (H P) - (P H^T) K^T + K S K^T,   S = H P H^T + R,
@@ -559,22 +625,129 @@ __device__ inline void update_fp64gain(Track& t, const Hit& h) {
 }
 
 // ---------------------------------------------------------------------------
+// KalmanUpdate v1 (OMP:375-501): 3-D hit, 3x3 inverse.  Off the reference's
+// path (commented out at OMP:743); the SYCL variant calls it.
+// ---------------------------------------------------------------------------
+
+// Word of (i, j) in the upper triangle stored row by row, the addressing v1
+// uses for both covariances ((0,0) (0,1) .. (0,5) (1,1) ..).
+__device__ constexpr int up_idx(int i, int j) {
+  return i <= j ? i * 6 - i * (i - 1) / 2 + (j - i) : j * 6 - j * (j - 1) / 2 + (i - j);
+}
+
+// v1 as written, including its addressing of the 21 covariance words and the
+// 6 hit covariance words as upper triangles (the path stores lower
+// triangles), its (0,2) inverse entry m01*m12 - m02*m12 (OMP:395) and its
+// write of pt + K*res into the ipt slot (OMP:447, 494).  Typing as written:
+// float sums and determinant expression, the determinant widened to double,
+// 1/det and its products in double, pt = 1./ipt in double; no contraction.
+// hc: the 6 hit covariance words.
+__device__ inline void update_v1_ref(Track& t, const float hc[6], float xout, float yout) {
+#pragma clang fp contract(off)
+  float* a = t.P;
+  float* par = t.p;
+  const float m00 = a[0] + hc[0], m01 = a[1] + hc[1], m02 = a[2] + hc[2];
+  const float m11 = a[6] + hc[3], m12 = a[7] + hc[4], m22 = a[11] + hc[5];
+  const double det = (double)((m00 * ((m11 * m22) - (m12 * m12))) - (m01 * ((m01 * m22) - (m12 * m02))) +
+                              (m02 * ((m01 * m12) - (m02 * m11))));
+  const double invdet = 1.0 / det;
+  float c[9];
+  c[0] = (float)(invdet * (double)((m11 * m22) - (m12 * m12)));
+  c[1] = (float)(-invdet * (double)((m01 * m22) - (m02 * m12)));
+  c[2] = (float)(invdet * (double)((m01 * m12) - (m02 * m12)));
+  c[3] = (float)(-invdet * (double)((m01 * m22) - (m12 * m02)));
+  c[4] = (float)(invdet * (double)((m00 * m22) - (m02 * m02)));
+  c[5] = (float)(-invdet * (double)((m00 * m12) - (m02 * m01)));
+  c[6] = (float)(invdet * (double)((m01 * m12) - (m02 * m11)));
+  c[7] = (float)(-invdet * (double)((m00 * m12) - (m02 * m01)));
+  c[8] = (float)(invdet * (double)((m00 * m11) - (m01 * m01)));
+  float K[18];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      K[3 * i + j] = a[up_idx(i, 0)] * c[j] + a[up_idx(i, 1)] * c[3 + j] + a[up_idx(i, 2)] * c[6 + j];
+  const float rx = xout - par[0], ry = yout - par[1];
+  const float ptin = (float)(1.0 / (double)par[3]);
+  float pn[6];
+  pn[0] = par[0] + (K[0] * rx) + (K[1] * ry);
+  pn[1] = par[1] + (K[3] * rx) + (K[4] * ry);
+  pn[2] = par[2] + (K[6] * rx) + (K[7] * ry);
+  pn[3] = ptin + (K[9] * rx) + (K[10] * ry);
+  pn[4] = par[4] + (K[12] * rx) + (K[13] * ry);
+  pn[5] = par[5] + (K[15] * rx) + (K[16] * ry);
+  float n[21];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = i; j < 6; ++j) {
+      const int e = up_idx(i, j);
+      n[e] = a[e] - (K[3 * i] * a[up_idx(0, j)] + K[3 * i + 1] * a[up_idx(1, j)] + K[3 * i + 2] * a[up_idx(2, j)]);
+    }
+#pragma unroll
+  for (int e = 0; e < 21; ++e) a[e] = n[e];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) par[e] = pn[e];
+}
+
+// The update v1 means, in the path's packing: lower triangles for both
+// covariances, residual (x, y, z) - (p0, p1, p2), S = P(0..2, 0..2) + R
+// inverted by cofactors in double, K = P H^T S^-1 in float, p' = p + K res
+// (ipt updated as ipt), P' = P - K H P.  No contraction.
+__device__ inline void update_v1_fixed(Track& t, const float hc[6], float hx, float hy, float hz) {
+#pragma clang fp contract(off)
+  float* P = t.P;
+  float* par = t.p;
+  const double m00 = P[S(0, 0)] + hc[0], m01 = P[S(1, 0)] + hc[1], m11 = P[S(1, 1)] + hc[2];
+  const double m02 = P[S(2, 0)] + hc[3], m12 = P[S(2, 1)] + hc[4], m22 = P[S(2, 2)] + hc[5];
+  const double c00 = m11 * m22 - m12 * m12, c01 = m02 * m12 - m01 * m22, c02 = m01 * m12 - m02 * m11;
+  const double c11 = m00 * m22 - m02 * m02, c12 = m01 * m02 - m00 * m12, c22 = m00 * m11 - m01 * m01;
+  const double det = m00 * c00 + m01 * c01 + m02 * c02;
+  const double id = 1.0 / det;
+  const float inv[3][3] = {{(float)(c00 * id), (float)(c01 * id), (float)(c02 * id)},
+                           {(float)(c01 * id), (float)(c11 * id), (float)(c12 * id)},
+                           {(float)(c02 * id), (float)(c12 * id), (float)(c22 * id)}};
+  float K[6][3];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) K[i][j] = P[S(i, 0)] * inv[0][j] + P[S(i, 1)] * inv[1][j] + P[S(i, 2)] * inv[2][j];
+  const float r0 = hx - par[0], r1 = hy - par[1], r2 = hz - par[2];
+  float n[21];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j)
+      n[S(i, j)] = P[S(i, j)] - (K[i][0] * P[S(0, j)] + K[i][1] * P[S(1, j)] + K[i][2] * P[S(2, j)]);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) par[i] = par[i] + K[i][0] * r0 + K[i][1] * r1 + K[i][2] * r2;
+#pragma unroll
+  for (int e = 0; e < 21; ++e) P[e] = n[e];
+}
+
+// ---------------------------------------------------------------------------
 // one layer
 // ---------------------------------------------------------------------------
 
 // kFirst: the covariance may still have a non-zero z row (first layer of a
 // track); otherwise the fast forms drop the structurally-zero terms.
+// g: sin/cos of phi and theta carried between the layers of one track
+// (FAST, FP64GAIN); set up on the first layer.
 template <int kMath, bool kFirst>
-__device__ inline void layer_step(Track& t, const Hit& h) {
+__device__ inline void layer_step(Track& t, const Hit& h, Trig& g) {
   if constexpr (kMath == kRef) {
     propagate_ref(t, h.z);
     update_ref(t, h);
   } else if constexpr (kMath == kFast) {
-    propagate_fast<!kFirst>(t, h.z);
-    update_fast<true>(t, h);
+    if constexpr (kFirst) trig_init<false>(t, g);
+    const Jac J = helix_core<kFirst>(t, h.z, g);
+    propagate_cov_fast<!kFirst>(t.P, J);
+    update_fast<true, true>(t, h, &g);
   } else {
-    propagate_fast<!kFirst, true>(t, h.z);
-    update_fp64gain<true>(t, h);
+    if constexpr (kFirst) trig_init<true>(t, g);
+    const Jac J = helix_core<kFirst>(t, h.z, g);
+    propagate_cov_fast<!kFirst>(t.P, J);
+    update_fp64gain<true, true>(t, h, &g);
   }
 }
 

@@ -8,6 +8,7 @@
 //   bTk, bHit and the par/pos accessors          (OMP:147-241)
 //   prepareTracks, prepareHits, randn            (OMP:127-145, 243-293)
 //   propagateToZ, KalmanUpdate_v2                (OMP:623, 503)
+//   KalmanUpdate (v1, off the reference's path)  (OMP:440)
 // Differences, all forced by moving from compile-time to runtime sizes:
 //   * the AoSoA width is the template argument W of the MP* types; the
 //     unqualified names use P2Z_BSIZE (default 32), the reference's bsize;
@@ -118,6 +119,12 @@ void propagateToZ(const MP6x6SF_<W>* inErr, const MP6F_<W>* inPar, const MP1I_<W
 template <int W>
 void KalmanUpdate_v2(MP6x6SF_<W>* trkErr, MP6F_<W>* inPar, const MP3x3SF_<W>* hitErr,
                      const MP3F_<W>* msP);
+// KalmanUpdate (v1, OMP:375-501; not called by the reference's driver): the
+// reference's code as written, or with fixed = true the textbook 3-D update
+// in the path's packing (p2z_kalman_update_v1).
+template <int W>
+void KalmanUpdate(MP6x6SF_<W>* trkErr, MP6F_<W>* inPar, const MP3x3SF_<W>* hitErr, const MP3F_<W>* msP,
+                  bool fixed = false);
 
 // The driver loop (OMP:731-748) for all events: outtrk = trk, then nlayer x
 // (propagateToZ, KalmanUpdate_v2) per block, on device-resident copies.

@@ -114,6 +114,31 @@ void KalmanUpdate_v2(MP6x6SF_<W>* trkErr, MP6F_<W>* inPar, const MP3x3SF_<W>* hi
 }
 
 template <int W>
+void KalmanUpdate(MP6x6SF_<W>* trkErr, MP6F_<W>* inPar, const MP3x3SF_<W>* hitErr, const MP3F_<W>* msP,
+                  bool fixed) {
+  p2z_ctx* ctx = p2zContext();
+  MPTRK_<W> t;
+  MPHIT_<W> h;
+  std::memcpy(&t.cov, trkErr, sizeof t.cov);
+  std::memcpy(&t.par, inPar, sizeof t.par);
+  std::memset(&t.q, 0, sizeof t.q);
+  std::memcpy(&h.pos, msP, sizeof h.pos);
+  std::memcpy(&h.cov, hitErr, sizeof h.cov);
+  DevBuf dt(sizeof t), dh(sizeof h);
+  p2zCheck(p2z_memcpy_h2d(ctx, dt.p, &t, sizeof t), "h2d");
+  p2zCheck(p2z_memcpy_h2d(ctx, dh.p, &h, sizeof h), "h2d");
+  auto* d = static_cast<MPTRK_<W>*>(dt.p);
+  auto* e = static_cast<MPHIT_<W>*>(dh.p);
+  p2zCheck(p2z_kalman_update_v1(ctx, d->cov.data, d->par.data, e->cov.data, e->pos.data, 1, 28 * W, 9 * W, W,
+                                fixed ? P2Z_V1_FIXED : 0u),
+           "KalmanUpdate");
+  p2zCheck(p2z_memcpy_d2h(ctx, &t, dt.p, sizeof t), "d2h");
+  p2zCheck(p2z_sync(ctx), "sync");
+  std::memcpy(trkErr, &t.cov, sizeof t.cov);
+  std::memcpy(inPar, &t.par, sizeof t.par);
+}
+
+template <int W>
 double p2zRunDriver(const MPTRK_<W>* trk, const MPHIT_<W>* hit, MPTRK_<W>* outtrk, int niter,
                     bool include_data) {
   const P2zConfig& c = p2zConfig();
@@ -149,6 +174,7 @@ double p2zRunDriver(const MPTRK_<W>* trk, const MPHIT_<W>* hit, MPTRK_<W>* outtr
   template void propagateToZ<W>(const MP6x6SF_<W>*, const MP6F_<W>*, const MP1I_<W>*,             \
                                 const MP3F_<W>*, MP6x6SF_<W>*, MP6F_<W>*);                        \
   template void KalmanUpdate_v2<W>(MP6x6SF_<W>*, MP6F_<W>*, const MP3x3SF_<W>*, const MP3F_<W>*); \
+  template void KalmanUpdate<W>(MP6x6SF_<W>*, MP6F_<W>*, const MP3x3SF_<W>*, const MP3F_<W>*, bool); \
   template double p2zRunDriver<W>(const MPTRK_<W>*, const MPHIT_<W>*, MPTRK_<W>*, int, bool);
 P2Z_INSTANTIATE(16)
 P2Z_INSTANTIATE(32)

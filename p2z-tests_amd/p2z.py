@@ -111,6 +111,7 @@ class ServeReport(C.Structure):
 
 
 RAGGED_PERSISTENT, RAGGED_GRAPH = 1, 2
+V1_FIXED = 1
 WORKLOAD = {"precision": 2, "ragged": 3}
 
 _lib = None
@@ -140,6 +141,7 @@ _SIGS = {
     "p2z_preset_input": ([i32, C.POINTER(ATRK), C.POINTER(AHIT), i32, C.POINTER(dbl)], i32),
     "p2z_propagate_to_z": ([vp, vp, vp, vp, vp, vp, vp, i64, i64, i64, i32, i32], i32),
     "p2z_kalman_update": ([vp, vp, vp, vp, vp, i64, i64, i64, i32, i32], i32),
+    "p2z_kalman_update_v1": ([vp, vp, vp, vp, vp, i64, i64, i64, i32, u32], i32),
     "p2z_run": ([vp, vp, vp, vp, i64, i64, i32, i32, i32], i32),
     "p2z_run_host": ([vp, vp, vp, vp, i64, i64, i32, i32, i32], i32),
     "p2z_run_ragged": ([vp, vp, vp, vp, vp, vp, i64, i32, i32, u32, C.POINTER(RaggedOpts)], i32),
@@ -568,6 +570,17 @@ class Context:
         f4 = 4
         check(lib().p2z_kalman_update(self.handle, trk.ptr + 6 * W * f4, trk.ptr, hit.ptr + 3 * W * f4, hit.ptr,
                                       nblocks, TRK_WORDS * W, hs, W, MATH.get(math, math)), "p2z_kalman_update")
+
+    def kalman_update_v1(self, trk: DeviceBuffer, hit: DeviceBuffer, nblocks: int, width: int,
+                         fixed: bool = False, hit_stride: int | None = None) -> None:
+        """KalmanUpdate v1 (3-D hit) in place on MPTRK blocks with one MPHIT
+        block each; fixed selects the textbook form (P2Z_V1_FIXED)."""
+        W = width
+        hs = HIT_WORDS * W if hit_stride is None else hit_stride
+        f4 = 4
+        check(lib().p2z_kalman_update_v1(self.handle, trk.ptr + 6 * W * f4, trk.ptr, hit.ptr + 3 * W * f4, hit.ptr,
+                                         nblocks, TRK_WORDS * W, hs, W, V1_FIXED if fixed else 0),
+              "p2z_kalman_update_v1")
 
     def run_ragged(self, trk, hit, out, lane_nlayer, blocks, nblocks: int, width: int, math="fast",
                    persistent: bool = True, graph: bool = False, tracks_per_wg: int = 0, prefetch: int = 0,

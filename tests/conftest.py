@@ -46,6 +46,7 @@ def golden():
         known_answer=lines("known_answer.txt"),
         ref_summary_full=lines("ref_summary_full.txt"),
         fp64_summary_full=lines("fp64_summary_full.txt"),
+        update_v1=npz("ref_update_v1.npz"),
     )
 
 

@@ -24,6 +24,8 @@ Files:
                         inputs: the answer without FP32 rounding
   ref_steps.npz         8 random blocks (bsize 32, seeded): one propagateToZ and
                         one KalmanUpdate_v2 each, from both reference builds
+  ref_update_v1.npz     8 random blocks (bsize 32, seeded, all 6 hit covariance
+                        words set): one KalmanUpdate (v1, OMP:375-501) each
   ref_spread.json       the reference's own per-track spread on the driver
                         outputs (small, w16 and mid = 10 events x 960 tracks)
                         and on the single steps, under
@@ -162,7 +164,24 @@ def main() -> None:
     with open(os.path.join(HERE, "ref_summary_full.txt"), "w") as f:
         f.write("\n".join(rep) + "\n")
     write_fp64_summary(t, h)
+    write_update_v1(need("small"))
     print("golden fixtures written to", HERE)
+
+
+def write_update_v1(lib) -> None:
+    from parity import v1_blocks
+    rng = np.random.default_rng(2027)
+    W = 32
+    blocks = [v1_blocks(rng, W) for _ in range(8)]
+    trk = np.concatenate([b[0] for b in blocks])
+    hit = np.concatenate([b[1] for b in blocks])
+    out = trk.copy()
+    for t in range(len(blocks)):
+        tb = np.ascontiguousarray(out[t * 28 * W:(t + 1) * 28 * W])
+        hb = np.ascontiguousarray(hit[t * 9 * W:(t + 1) * 9 * W])
+        lib.ref_update_v1_block(tb.ctypes.data, hb.ctypes.data)
+        out[t * 28 * W:(t + 1) * 28 * W] = tb
+    np.savez_compressed(os.path.join(HERE, "ref_update_v1.npz"), trk=trk, hit=hit, out=out, width=np.int64(W))
 
 
 def write_fp64_summary(t, h) -> None:

@@ -102,3 +102,14 @@ def random_blocks(rng, W: int, zrow: bool = True):
                           rng.uniform(-1e-5, 1e-5, n), rng.uniform(1e-4, 1e-1, n),
                           np.zeros(3 * n)]).astype(np.float32)
     return blk, hit
+
+
+def v1_blocks(rng, W: int):
+    """random_blocks with all 6 hit covariance words set (xx, xy, yy, xz, yz,
+    zz in the path's lower packing), for the 3-D update v1."""
+    blk, hit = random_blocks(rng, W)
+    n = W
+    hit[6 * n:7 * n] = rng.uniform(1e-6, 1e-5, n)   # xz
+    hit[7 * n:8 * n] = rng.uniform(1e-6, 1e-5, n)   # yz
+    hit[8 * n:9 * n] = rng.uniform(1e-2, 1e-1, n)   # zz
+    return blk, hit

@@ -364,3 +364,99 @@ def test_ragged_driver_equals_plain_driver(golden):
             a = out.reshape(nblk, 28, W)[b, :, l]
             e = plain.reshape(nblk, 28, W)[b, :, l]
             assert np.array_equal(a.view(np.uint32), e.view(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# KalmanUpdate v1 (OMP:375-501), the optional 3-D update
+# ---------------------------------------------------------------------------
+
+def _v1_oracle(trk, hit, W, fixed):
+    nblk = len(trk) // (28 * W)
+    return np.concatenate([O.update_v1_block(np.ascontiguousarray(trk[b * 28 * W:(b + 1) * 28 * W]),
+                                              np.ascontiguousarray(hit[b * 9 * W:(b + 1) * 9 * W]), W, fixed)
+                           for b in range(nblk)])
+
+
+def test_update_v1_matches_reference_build(golden):
+    """The oracle's v1 as written is the reference's KalmanUpdate word for
+    word (fixture written by the reference build, tests/golden/make_golden.py),
+    and the reference build itself where it is present."""
+    g = golden["update_v1"]
+    W = int(g["width"])
+    mine = _v1_oracle(g["trk"], g["hit"], W, False)
+    assert np.array_equal(mine.view(np.uint32), g["out"].view(np.uint32))
+    L = O.ref_lib("small")
+    if L is not None:
+        from parity import v1_blocks
+        rng = np.random.default_rng(8)
+        for _ in range(4):
+            t, h = v1_blocks(rng, W)
+            ref = t.copy()
+            L.ref_update_v1_block(ref.ctypes.data, h.ctypes.data)
+            assert np.array_equal(O.update_v1_block(t, h, W, False).view(np.uint32), ref.view(np.uint32))
+
+
+def _sym(c):
+    P = np.zeros((6, 6))
+    k = 0
+    for i in range(6):
+        for j in range(i + 1):
+            P[i, j] = P[j, i] = c[k]
+            k += 1
+    return P
+
+
+def test_update_v1_fixed_is_textbook():
+    """The fixed form is the textbook 3-D Kalman update K = P H^T (H P H^T +
+    R)^-1, p' = p + K (m - H p), P' = P - K H P with H = [I3 0], computed
+    here independently in numpy double."""
+    from parity import v1_blocks
+    rng = np.random.default_rng(21)
+    W = 32
+    for _ in range(4):
+        trk, hit = v1_blocks(rng, W)
+        out = O.update_v1_block(trk, hit, W, True)
+        for l in range(W):
+            par = np.array([trk[e * W + l] for e in range(6)], np.float64)
+            P = _sym([trk[(6 + e) * W + l] for e in range(21)])
+            hc = [hit[(3 + e) * W + l] for e in range(6)]
+            R = np.array([[hc[0], hc[1], hc[3]], [hc[1], hc[2], hc[4]], [hc[3], hc[4], hc[5]]], np.float64)
+            m = np.array([hit[e * W + l] for e in range(3)], np.float64)
+            H = np.zeros((3, 6))
+            H[0, 0] = H[1, 1] = H[2, 2] = 1
+            K = P @ H.T @ np.linalg.inv(H @ P @ H.T + R)
+            p2 = par + K @ (m - H @ par)
+            P2 = P - K @ H @ P
+            gp = np.array([out[e * W + l] for e in range(6)], np.float64)
+            gP = _sym([out[(6 + e) * W + l] for e in range(21)])
+            assert np.all(np.abs(gp - p2) <= 1e-5 * np.maximum(np.abs(p2), 1.0)), (l, gp, p2)
+            d = np.sqrt(np.abs(np.diag(P2)))
+            scale = np.maximum(np.abs(P2), 1e-3 * np.outer(d, d))
+            assert np.max(np.abs(gP - P2) / scale) < 1e-3, l
+
+
+def test_update_v1_as_written_writes_pt_into_ipt():
+    """The defect the as-written form carries (OMP:447, 494): with an
+    uninformative hit (R -> infinity, K -> 0) it leaves x, y, z, phi, theta
+    and P as they are but replaces ipt by pt = 1/ipt; the fixed form leaves
+    ipt too."""
+    from parity import random_blocks
+    rng = np.random.default_rng(4)
+    W = 16
+    trk, hit = random_blocks(rng, W)
+    big = np.float32(1e10)                    # the float determinant overflows past ~1e12
+    up = hit.copy()                           # upper addressing: xx xy xz yy yz zz
+    up[3 * W:9 * W] = 0
+    up[3 * W:4 * W] = up[6 * W:7 * W] = up[8 * W:9 * W] = big
+    lo = hit.copy()                           # lower packing: xx xy yy xz yz zz
+    lo[3 * W:9 * W] = 0
+    lo[3 * W:4 * W] = lo[5 * W:6 * W] = lo[8 * W:9 * W] = big
+    a = O.update_v1_block(trk, up, W, False)
+    f = O.update_v1_block(trk, lo, W, True)
+    for out in (a, f):
+        for e in (0, 1, 2, 4, 5):
+            assert np.allclose(out[e * W:(e + 1) * W], trk[e * W:(e + 1) * W], rtol=1e-6, atol=1e-6)
+        assert np.allclose(out[6 * W:27 * W], trk[6 * W:27 * W], rtol=1e-5, atol=1e-12)
+    ipt = trk[3 * W:4 * W].astype(np.float64)
+    assert np.allclose(a[3 * W:4 * W], 1.0 / ipt, rtol=1e-6)
+    assert np.allclose(f[3 * W:4 * W], ipt, rtol=1e-6)
