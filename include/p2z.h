@@ -57,7 +57,8 @@ typedef struct p2z_ctx p2z_ctx;
 typedef struct {
   int32_t tracks_per_wg; /* workgroup size in tracks (64..1024, multiple of 64); 0 = 256 */
   int32_t nstreams;      /* streams of the host-buffer pipeline; 0 = 4            */
-  int32_t hit_stage;     /* LDS hit-staging depth in layers; 0 = register prefetch */
+  int32_t hit_stage;     /* hit-staging depth: layers of hits in flight ahead of
+                            the one being computed, 1 or 2; 0 = default (2)  */
   int32_t reserved;
 } p2z_opts;
 
@@ -166,13 +167,15 @@ typedef struct {
                                     (descriptor order = processing order);
                                     otherwise one wave per 64 track slots     */
 #define P2Z_RAGGED_GRAPH 2u      /* launch through a captured hipGraph, cached
-                                    per argument set (persistent form only)   */
+                                    per argument set (either form)            */
 
 typedef struct {
   int32_t tracks_per_wg; /* 0: the context's setting                          */
-  int32_t prefetch;      /* hit layers loaded ahead: 1 or 2; 0 = 1             */
+  int32_t prefetch;      /* hit layers loaded ahead: 1 or 2; 0 = 2             */
   int32_t waves_per_cu;  /* persistent grid = CUs * waves_per_cu waves; 0 = 8  */
-  int32_t reserved;
+  int32_t items_per_claim; /* persistent form: work items (64 track slots
+                              each) a wave takes per atomic on the queue
+                              counter, 1..64; 0 = 4                          */
 } p2z_ragged_opts;
 
 /* Runs every block of the descriptor table (device memory) through its layers.
@@ -256,7 +259,7 @@ typedef struct {
   int32_t nstreams;         /* batches in flight at once, 1..16; 0 = 4         */
   int32_t max_batch_blocks; /* block limit of one batch; 0 = none              */
   int32_t width, math;      /* AoSoA width of the pool, arithmetic mode        */
-  int32_t tracks_per_wg, prefetch, waves_per_cu, reserved; /* as p2z_ragged_opts */
+  int32_t tracks_per_wg, prefetch, waves_per_cu, items_per_claim; /* as p2z_ragged_opts */
 } p2z_server_opts;
 
 typedef struct {

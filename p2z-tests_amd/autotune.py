@@ -1,8 +1,8 @@
 """Autotuner of the persistent ragged launch.
 
-Sweeps the AoSoA width W in {16, 32, 64, 128}, tracks per workgroup, hit
-prefetch depth and the persistent grid size (waves per CU) on a sample of the
-workload, timing each candidate with hipEvents (median of repeats), and
+Sweeps the launch form (persistent work queue or one-shot), the AoSoA width
+W in {16, 32, 64, 128}, tracks per workgroup, hit prefetch depth and, for the
+persistent form, the grid size (waves per CU) on a sample of the workload, timing each candidate with hipEvents (median of repeats), and
 returns the fastest.  The sample is generated at each width by the same
 generator (the tracks do not depend on W, so every candidate does the same
 work).  Results are cached per device and workload kind in
@@ -24,9 +24,10 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import p2z  # noqa: E402
 
 WIDTHS = (16, 32, 64, 128)
-TPWS = (64, 128, 256)
+TPWS = (64, 256)
 PREFETCH = (1, 2)
-WAVES_PER_CU = (4, 8, 12, 16, 24, 32)
+WAVES_PER_CU = (8, 16, 24, 32)
+CLAIMS = (2, 4, 8)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "p2z", "autotune.json")
 
 
@@ -42,8 +43,13 @@ def _time(ctx, fn, reps: int) -> float:
 
 
 def tune(ctx, kind: str = "ragged", nevents: int = 200, math: str = "fp64gain", seed: int = 17,
-         reps: int = 5, widths=WIDTHS, tpws=TPWS, prefetch=PREFETCH, waves=WAVES_PER_CU,
-         log=lambda s: None) -> dict:
+         reps: int = 5, widths=WIDTHS, tpws=TPWS, prefetch=PREFETCH, waves=WAVES_PER_CU, claims=CLAIMS,
+         forms=("persistent", "oneshot"), log=lambda s: None) -> dict:
+    """Times every candidate launch on the sample and returns the fastest
+    overall (best) and the fastest of each launch form (best_<form>).  The
+    persistent form sweeps tracks per workgroup x prefetch depth x waves per
+    CU x items per claim; the one-shot form (one wave per 64 track slots, dispatched by the
+    hardware) sweeps tracks per workgroup x prefetch depth."""
     results = []
     for W in widths:
         w = p2z.Workload(kind, nevents=nevents, width=W, seed=seed)
@@ -51,19 +57,32 @@ def tune(ctx, kind: str = "ragged", nevents: int = 200, math: str = "fp64gain", 
         lanes = ctx.upload(np.ascontiguousarray(w.lane_nlayer))
         blk = ctx.upload(np.ascontiguousarray(w.blocks_lpt))
         out = ctx.alloc(w.tracks.nbytes)
-        for tpw, pf, wpc in itertools.product(tpws, prefetch, waves):
-            ms = _time(ctx, lambda: ctx.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, math,
-                                                   tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc), reps)
-            rec = dict(width=W, tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc, ms=ms,
-                       tl_per_s=w.track_layers / (ms * 1e-3))
+        cands = []
+        if "persistent" in forms:
+            cands += [("persistent", tpw, pf, wpc, cl)
+                      for tpw, pf, wpc, cl in itertools.product(tpws, prefetch, waves, claims)]
+        if "oneshot" in forms:
+            cands += [("oneshot", tpw, pf, 0, 0) for tpw, pf in itertools.product(tpws, prefetch)]
+        for form, tpw, pf, wpc, cl in cands:
+            pers = form == "persistent"
+            ms = _time(ctx, lambda: ctx.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, math, persistent=pers,
+                                                   tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc,
+                                                   items_per_claim=cl), reps)
+            rec = dict(form=form, width=W, tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc, items_per_claim=cl,
+                       ms=ms, tl_per_s=w.track_layers / (ms * 1e-3))
             results.append(rec)
-            log(f"  W={W:3d} tpw={tpw:3d} prefetch={pf} waves/CU={wpc:2d}: {ms:.4f} ms  {rec['tl_per_s']:.3e} TL/s")
+            log(f"  {form:10s} W={W:3d} tpw={tpw:3d} prefetch={pf} waves/CU={wpc:2d} claim={cl}: {ms:.4f} ms  "
+                f"{rec['tl_per_s']:.3e} TL/s")
         for b in (trk, hit, lanes, blk, out):
             b.free()
         w.close()
-    best = min(results, key=lambda r: r["ms"])
-    return dict(best=best, kind=kind, math=math, sample_events=nevents, candidates=len(results),
-                results=results)
+    res = dict(best=min(results, key=lambda r: r["ms"]), kind=kind, math=math, sample_events=nevents,
+               candidates=len(results), results=results)
+    for form in forms:
+        rs = [r for r in results if r["form"] == form]
+        if rs:
+            res["best_" + form] = min(rs, key=lambda r: r["ms"])
+    return res
 
 
 def cached_or_tune(ctx, key: str, cache: bool = True, **kw) -> dict:

@@ -44,9 +44,9 @@ int apply_opts(p2z_ctx* c, const p2z_opts* o) {
         return p2z::fail(P2Z_EINVAL, "nstreams must be in [1,16] (got %d)", o->nstreams);
       ns = o->nstreams;
     }
-    if (o->hit_stage < 0 || o->hit_stage > 8)
-      return p2z::fail(P2Z_EINVAL, "hit_stage must be in [0,8] (got %d)", o->hit_stage);
-    cfg.hit_stage = o->hit_stage;
+    if (o->hit_stage < 0 || o->hit_stage > 2)
+      return p2z::fail(P2Z_EINVAL, "hit_stage must be 1 or 2, or 0 for the default (got %d)", o->hit_stage);
+    if (o->hit_stage) cfg.hit_stage = o->hit_stage;
   }
   c->cfg = cfg;
   if ((int)c->pipe.size() != ns) {
@@ -347,9 +347,7 @@ int p2z_run_ragged(p2z_ctx* c, const void* trk, const void* hit, void* out, cons
   if (nblocks < 0 || nblocks * (int64_t)width > ((int64_t)1 << 31) - 64)
     return p2z::fail(P2Z_EINVAL, "bad block count %lld", (long long)nblocks);
   if (flags & ~(P2Z_RAGGED_PERSISTENT | P2Z_RAGGED_GRAPH)) return p2z::fail(P2Z_EINVAL, "unknown flags 0x%x", flags);
-  if ((flags & P2Z_RAGGED_GRAPH) && !(flags & P2Z_RAGGED_PERSISTENT))
-    return p2z::fail(P2Z_EINVAL, "P2Z_RAGGED_GRAPH needs P2Z_RAGGED_PERSISTENT");
-  int tpw = c->cfg.tracks_per_wg, prefetch = 1, wpc = 8;
+  int tpw = c->cfg.tracks_per_wg, prefetch = 2, wpc = 8, claim = p2z::kDefaultClaim;
   if (opts) {
     if (opts->tracks_per_wg) {
       if (opts->tracks_per_wg < 64 || opts->tracks_per_wg > 256 || opts->tracks_per_wg % 64)
@@ -364,6 +362,11 @@ int p2z_run_ragged(p2z_ctx* c, const void* trk, const void* hit, void* out, cons
       if (opts->waves_per_cu < 1 || opts->waves_per_cu > 32) return p2z::fail(P2Z_EINVAL, "waves_per_cu in [1,32]");
       wpc = opts->waves_per_cu;
     }
+    if (opts->items_per_claim) {
+      if (opts->items_per_claim < 1 || opts->items_per_claim > 64)
+        return p2z::fail(P2Z_EINVAL, "items_per_claim in [1,64]");
+      claim = opts->items_per_claim;
+    }
   }
   P2Z_TRY(use_device(c));
   if (nblocks == 0) return P2Z_OK;
@@ -372,14 +375,16 @@ int p2z_run_ragged(p2z_ctx* c, const void* trk, const void* hit, void* out, cons
   const auto* bd = reinterpret_cast<const p2z::BlockDesc*>(blocks);
   auto launch = [&](hipStream_t s) {
     return p2z::launch_ragged((const float*)trk, (const float*)hit, (float*)out, lane_nlayer, bd, nblocks, width,
-                              math, persistent, prefetch, tpw, grid, c->counter, s);
+                              math, persistent, prefetch, tpw, grid, claim, c->counter, s);
   };
   if (!(flags & P2Z_RAGGED_GRAPH)) {
     P2Z_HIP_CHECK(launch(c->stream));
     return P2Z_OK;
   }
-  // counter reset + persistent kernel, captured once per argument set
-  const p2z_ctx::GraphKey key{trk, hit, out, lane_nlayer, blocks, nblocks, width, math, prefetch, tpw, grid};
+  // the launch (counter reset + persistent kernel, or the one-shot kernel),
+  // captured once per argument set
+  const p2z_ctx::GraphKey key{trk, hit, out, lane_nlayer, blocks, nblocks, width, math, prefetch, tpw,
+                              persistent ? grid : -1, claim};
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     hipGraph_t g = nullptr;

@@ -25,7 +25,7 @@ struct p2z_ctx {
   unsigned* counter = nullptr;
   int cus = 0;
   using GraphKey = std::tuple<const void*, const void*, void*, const void*, const void*, int64_t, int, int, int,
-                              int, int64_t>;
+                              int, int64_t, int>;
   std::map<GraphKey, hipGraphExec_t> graphs;
 };
 

@@ -15,7 +15,7 @@ inline bool valid_math(int m) { return m >= 0 && m <= 2; }
 
 struct LaunchCfg {
   int tracks_per_wg = 256;  // threads per workgroup, one track per thread
-  int hit_stage = 0;        // LDS staging depth in layers (0: register prefetch)
+  int hit_stage = 2;        // layers of hits in flight ahead of the computed one (1 or 2)
 };
 
 // Fused driver pass: ntracks = nblocks*W tracks, all with nlayer layers.
@@ -46,11 +46,12 @@ struct BlockDesc {
 static_assert(sizeof(BlockDesc) == 32, "descriptor layout");
 
 // Ragged pass.  persistent: grid workgroups drain the descriptor table through
-// the device counter (reset first on the same stream); otherwise one wave per
-// 64 track slots.
+// the device counter (reset first on the same stream), claim items per atomic;
+// otherwise one wave per 64 track slots.
 hipError_t launch_ragged(const float* trk, const float* hit, float* out, const int32_t* lane_nlay,
                          const BlockDesc* blocks, int64_t nblocks, int width, int math, bool persistent,
-                         int prefetch, int tpw, int64_t grid, unsigned* counter, hipStream_t s);
+                         int prefetch, int tpw, int64_t grid, int claim, unsigned* counter, hipStream_t s);
+constexpr int kDefaultClaim = 4;
 
 // Report statistics (OMP:758-851) on device.  part: scratch of
 // summary_scratch_doubles(nevts, ntrks) doubles.  res: 18 doubles on device

@@ -47,31 +47,90 @@ __device__ inline Hit load_hit(const float* __restrict__ hb) {
 }
 
 // Layers [0, n) of one track whose hits start at hb (consecutive layers 9W
-// words apart), with the next layer's hit in flight while a layer is
-// computed.  Unrolled by two over alternating hit buffers, so the prefetched
-// hit is never copied between registers.  Every load index is < n.
-template <int W, int kMath>
-__device__ inline void run_layers(Track& t, const float* __restrict__ hb, int n) {
-  if (n <= 0) return;
+// words apart), with the hits of the next kDepth layers (1 or 2) in flight
+// while a layer is computed.  The loop is unrolled by kDepth + 1 over
+// rotating hit buffers, so a prefetched hit is never copied between
+// registers.  The loop runs to nmax, the same for every lane of the wave
+// (the fused pass: n == nmax); with kPerLane, lane layer counts n <= nmax
+// differ and each lane's loads and steps stop at its own n.  Every load index
+// is < n.
+template <int W, int kMath, int kDepth, bool kPerLane>
+__device__ __forceinline__ void run_layers(Track& t, const float* __restrict__ hb, int n, int nmax) {
+  static_assert(kDepth == 1 || kDepth == 2, "prefetch depth 1 or 2");
+  if (nmax <= 0) return;
+  auto at = [&](int lay) { return load_hit<W>(hb + (int64_t)lay * (kHitWords * W)); };
+  auto live = [&](int lay) { return kPerLane ? lay < n : true; };
   Trig g;
-  Hit a = load_hit<W>(hb);
-  Hit b = a;
-  if (n > 1) b = load_hit<W>(hb + kHitWords * W);
-  layer_step<kMath, true>(t, a, g);
-  int lay = 1;
-  for (; lay + 1 < n; lay += 2) {  // b holds layer lay
-    a = load_hit<W>(hb + (int64_t)(lay + 1) * (kHitWords * W));
-    layer_step<kMath, false>(t, b, g);
-    if (lay + 2 < n) b = load_hit<W>(hb + (int64_t)(lay + 2) * (kHitWords * W));
-    layer_step<kMath, false>(t, a, g);
+  Hit a{}, b{};
+  if (live(0)) a = at(0);
+  if (1 < nmax && live(1)) b = at(1);
+  if constexpr (kDepth == 1) {
+    if (live(0)) layer_step<kMath, true>(t, a, g);
+    int lay = 1;
+    for (; lay + 1 < nmax; lay += 2) {  // b holds layer lay
+      if (live(lay + 1)) a = at(lay + 1);
+      if (live(lay)) layer_step<kMath, false>(t, b, g);
+      if (lay + 2 < nmax && live(lay + 2)) b = at(lay + 2);
+      if (live(lay + 1)) layer_step<kMath, false>(t, a, g);
+    }
+    if (lay < nmax && live(lay)) layer_step<kMath, false>(t, b, g);
+  } else {
+    Hit c{};
+    if (2 < nmax && live(2)) c = at(2);
+    if (live(0)) layer_step<kMath, true>(t, a, g);
+    int lay = 1;
+    for (; lay + 2 < nmax; lay += 3) {  // b holds layer lay, c layer lay + 1
+      if (live(lay + 2)) a = at(lay + 2);
+      if (live(lay)) layer_step<kMath, false>(t, b, g);
+      if (lay + 3 < nmax && live(lay + 3)) b = at(lay + 3);
+      if (live(lay + 1)) layer_step<kMath, false>(t, c, g);
+      if (lay + 4 < nmax && live(lay + 4)) c = at(lay + 4);
+      if (live(lay + 2)) layer_step<kMath, false>(t, a, g);
+    }
+    if (lay < nmax && live(lay)) layer_step<kMath, false>(t, b, g);
+    if (lay + 1 < nmax && live(lay + 1)) layer_step<kMath, false>(t, c, g);
   }
-  if (lay < n) layer_step<kMath, false>(t, b, g);
+}
+
+// Ragged form of run_layers: lane layer counts n <= nmax differ, the loop
+// runs to nmax (the same for every lane of the wave) with one layer step per
+// iteration, each lane's loads and steps masked by its own n.  The hits of the
+// next kDepth layers are in flight while a layer is computed; a single step
+// instance keeps the register count down where the steps are masked.
+template <int W, int kMath, int kDepth>
+__device__ __forceinline__ void run_layers_lanes(Track& t, const float* __restrict__ hb, int n, int nmax) {
+  static_assert(kDepth == 1 || kDepth == 2, "prefetch depth 1 or 2");
+  if (nmax <= 0) return;
+  auto at = [&](int lay) { return load_hit<W>(hb + (int64_t)lay * (kHitWords * W)); };
+  Trig g;
+  Hit h0{}, h1{}, h2{};
+  if (0 < n) h0 = at(0);
+  if (1 < n) h1 = at(1);
+  if (kDepth >= 2 && 2 < n) h2 = at(2);
+  if (0 < n) layer_step<kMath, true>(t, h0, g);
+  for (int lay = 1; lay < nmax; ++lay) {
+    h0 = h1;
+    if constexpr (kDepth >= 2) {
+      h1 = h2;
+      if (lay + 2 < n) h2 = at(lay + 2);
+    } else if (lay + 1 < n) {
+      h1 = at(lay + 1);
+    }
+    if (lay < n) layer_step<kMath, false>(t, h0, g);
+  }
+}
+
+// Maximum of v over the 64 lanes of the wave (every lane must be active).
+__device__ inline int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
 // ---------------------------------------------------------------------------
 // fused driver pass (reference OMP:733-747 for every block at once)
 // ---------------------------------------------------------------------------
-template <int W, int kMath>
+template <int W, int kMath, int kDepth>
 __global__ __launch_bounds__(256) void k_fused(const float* __restrict__ trk,
                                                const float* __restrict__ hit,
                                                float* __restrict__ out, int64_t ntracks,
@@ -82,7 +141,7 @@ __global__ __launch_bounds__(256) void k_fused(const float* __restrict__ trk,
   const int l = (int)(g % W);
   Track t;
   load_track<W>(t, trk + b * (kTrkWords * W) + l);
-  run_layers<W, kMath>(t, hit + b * (int64_t)nlayer * (kHitWords * W) + l, nlayer);
+  run_layers<W, kMath, kDepth, false>(t, hit + b * (int64_t)nlayer * (kHitWords * W) + l, nlayer, nlayer);
   store_track<W>(t, out + b * (kTrkWords * W) + l);
 }
 
@@ -90,76 +149,67 @@ __global__ __launch_bounds__(256) void k_fused(const float* __restrict__ trk,
 // ragged events: per-block descriptors, per-lane layer counts
 // ---------------------------------------------------------------------------
 
-// One track of a descriptor's block through its own layer count, with the
-// hits of the next kPrefetch layers in flight while a layer is computed.
-template <int W, int kMath, int kPrefetch>
-__device__ inline void ragged_track(const float* __restrict__ trk, const float* __restrict__ hit,
-                                    float* __restrict__ out, const int32_t* __restrict__ lane_nlay,
-                                    const BlockDesc& d, int l) {
-  Track t;
-  load_track<W>(t, trk + d.trk_block * (kTrkWords * W) + l);
-  int n = lane_nlay[d.trk_block * W + l];
-  n = n < d.nlayer ? n : d.nlayer;
-  const float* hb = hit + d.hit_block * (kHitWords * W) + l;
-  if constexpr (kPrefetch < 2) {
-    run_layers<W, kMath>(t, hb, n);
-  } else if (n > 0) {
-    Trig g;
-    Hit h0 = load_hit<W>(hb), h1 = h0, h2 = h0;
-    if (n > 1) h1 = load_hit<W>(hb + kHitWords * W);
-    if (n > 2) h2 = load_hit<W>(hb + 2 * kHitWords * W);
-    layer_step<kMath, true>(t, h0, g);
-    for (int lay = 1; lay < n; ++lay) {
-      h0 = h1;
-      h1 = h2;
-      if (lay + 2 < n) h2 = load_hit<W>(hb + (int64_t)(lay + 2) * (kHitWords * W));
-      layer_step<kMath, false>(t, h0, g);
-    }
-  }
-  store_track<W>(t, out + d.out_block * (kTrkWords * W) + l);
-}
-
 // Work item = 64 consecutive track slots of the descriptor table (one wave):
-// slot s is lane s % W of descriptor s / W.
+// slot s is lane s % W of descriptor s / W.  Each lane runs its own layer
+// count; the wave's loop runs to the largest.  All 64 lanes of the wave must
+// be active (slots past the table take part with no layers).
 template <int W, int kMath, int kPrefetch>
-__device__ inline void ragged_item(const float* trk, const float* hit, float* out, const int32_t* lane_nlay,
-                                   const BlockDesc* __restrict__ blocks, int64_t nblocks, int64_t item,
-                                   int lane) {
+__device__ __forceinline__ void ragged_item(const float* __restrict__ trk, const float* __restrict__ hit,
+                                   float* __restrict__ out, const int32_t* __restrict__ lane_nlay,
+                                   const BlockDesc* __restrict__ blocks, int64_t nblocks, int64_t item, int lane) {
   const int64_t slot = item * 64 + lane;
   const int64_t di = slot / W;
-  if (di >= nblocks) return;
-  const BlockDesc d = blocks[di];
-  ragged_track<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, d, (int)(slot % W));
+  const int l = (int)(slot % W);
+  const bool valid = di < nblocks;
+  Track t{};
+  int n = 0;
+  const float* hb = hit;
+  BlockDesc d{};
+  if (valid) {
+    d = blocks[di];
+    load_track<W>(t, trk + d.trk_block * (kTrkWords * W) + l);
+    n = lane_nlay[d.trk_block * W + l];
+    n = n < d.nlayer ? n : d.nlayer;
+    n = n > 0 ? n : 0;
+    hb = hit + d.hit_block * (kHitWords * W) + l;
+  }
+  run_layers_lanes<W, kMath, kPrefetch>(t, hb, n, wave_max(n));
+  if (valid) store_track<W>(t, out + d.out_block * (kTrkWords * W) + l);
 }
 
-// Persistent form: each wave takes items from a device counter until it is
-// handed one past the end (every wave reaches that exit).  A wave claims its
-// next item before computing the current one, so the atomic's round trip
-// overlaps the computation; every item below nitems is claimed exactly once
-// and so computed exactly once.  The counter starts at 0 (k_reset_counter runs
-// first on the same stream).
+// Persistent form: each wave takes chunks of `claim` consecutive items from a
+// device counter until it is handed a chunk past the end (every wave reaches
+// that exit).  A wave claims its next chunk before computing the current one,
+// so the atomic's round trip overlaps the computation; every chunk is claimed
+// exactly once, so every item is computed exactly once.  Several items per
+// claim keep the single counter's atomics (serialised where they meet) few.
+// The counter starts at 0 (k_reset_counter runs first on the same stream).
 template <int W, int kMath, int kPrefetch>
 __global__ __launch_bounds__(256) void k_ragged_persistent(const float* __restrict__ trk,
                                                            const float* __restrict__ hit,
                                                            float* __restrict__ out,
                                                            const int32_t* __restrict__ lane_nlay,
                                                            const BlockDesc* __restrict__ blocks,
-                                                           int64_t nblocks, unsigned* __restrict__ counter) {
+                                                           int64_t nblocks, unsigned* __restrict__ counter,
+                                                           int claim) {
   const int64_t nitems = (nblocks * W + 63) / 64;
   const int lane = threadIdx.x & 63;
   unsigned mine = 0;
   if (lane == 0) mine = atomicAdd(counter, 1u);
-  int64_t item = (int64_t)(unsigned)__shfl((int)mine, 0);
-  while (item < nitems) {
+  int64_t first = (int64_t)(unsigned)__shfl((int)mine, 0) * claim;
+  while (first < nitems) {
     unsigned next = 0;
     if (lane == 0) next = atomicAdd(counter, 1u);
-    ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, lane);
-    item = (int64_t)(unsigned)__shfl((int)next, 0);
+    const int64_t last = first + claim < nitems ? first + claim : nitems;
+    for (int64_t item = first; item < last; ++item)
+      ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, lane);
+    first = (int64_t)(unsigned)__shfl((int)next, 0) * claim;
   }
 }
 
-// One-shot form: one wave per item.
-template <int W, int kMath>
+// One-shot form: one wave per item (the hardware dispatcher hands out the
+// items, in descriptor order).
+template <int W, int kMath, int kPrefetch>
 __global__ __launch_bounds__(256) void k_ragged_oneshot(const float* __restrict__ trk,
                                                         const float* __restrict__ hit,
                                                         float* __restrict__ out,
@@ -169,7 +219,7 @@ __global__ __launch_bounds__(256) void k_ragged_oneshot(const float* __restrict_
   const int64_t nitems = (nblocks * W + 63) / 64;
   const int64_t item = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   if (item >= nitems) return;
-  ragged_item<W, kMath, 1>(trk, hit, out, lane_nlay, blocks, nblocks, item, threadIdx.x & 63);
+  ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, threadIdx.x & 63);
 }
 
 __global__ void k_reset_counter(unsigned* counter) {
@@ -430,8 +480,12 @@ hipError_t launch_fused(const float* trk, const float* hit, float* out, int64_t 
   return dispatch_width(width, [&](auto wc) {
     return dispatch_math(math, [&](auto mc) {
       constexpr int W = decltype(wc)::value, M = decltype(mc)::value;
-      hipLaunchKernelGGL((k_fused<W, M>), dim3((unsigned)grid_for(ntracks, tpw)), dim3(tpw), 0, s,
-                         trk, hit, out, ntracks, nlayer);
+      if (cfg.hit_stage == 1)
+        hipLaunchKernelGGL((k_fused<W, M, 1>), dim3((unsigned)grid_for(ntracks, tpw)), dim3(tpw), 0, s,
+                           trk, hit, out, ntracks, nlayer);
+      else
+        hipLaunchKernelGGL((k_fused<W, M, 2>), dim3((unsigned)grid_for(ntracks, tpw)), dim3(tpw), 0, s,
+                           trk, hit, out, ntracks, nlayer);
       return hipGetLastError();
     });
   });
@@ -489,27 +543,32 @@ hipError_t launch_update_v1(float* trkErr, float* inPar, const float* hitErr, co
 
 hipError_t launch_ragged(const float* trk, const float* hit, float* out, const int32_t* lane_nlay,
                          const BlockDesc* blocks, int64_t nblocks, int width, int math, bool persistent,
-                         int prefetch, int tpw, int64_t grid, unsigned* counter, hipStream_t s) {
+                         int prefetch, int tpw, int64_t grid, int claim, unsigned* counter, hipStream_t s) {
   const int64_t nitems = (nblocks * width + 63) / 64;
   if (nitems == 0) return hipSuccess;
   return dispatch_width(width, [&](auto wc) {
     return dispatch_math(math, [&](auto mc) {
       constexpr int W = decltype(wc)::value, M = decltype(mc)::value;
       if (!persistent) {
-        hipLaunchKernelGGL((k_ragged_oneshot<W, M>), dim3((unsigned)grid_for(nitems * 64, tpw)), dim3(tpw), 0, s,
-                           trk, hit, out, lane_nlay, blocks, nblocks);
+        const dim3 grid1((unsigned)grid_for(nitems * 64, tpw));
+        if (prefetch >= 2)
+          hipLaunchKernelGGL((k_ragged_oneshot<W, M, 2>), grid1, dim3(tpw), 0, s, trk, hit, out, lane_nlay, blocks,
+                             nblocks);
+        else
+          hipLaunchKernelGGL((k_ragged_oneshot<W, M, 1>), grid1, dim3(tpw), 0, s, trk, hit, out, lane_nlay, blocks,
+                             nblocks);
         return hipGetLastError();
       }
       hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, counter);
-      const int64_t items_per_wg = tpw / 64;
+      const int64_t items_per_wg = (int64_t)(tpw / 64) * claim;
       int64_t g = grid > 0 ? grid : 1;
       g = std::min<int64_t>(g, (nitems + items_per_wg - 1) / items_per_wg);
       if (prefetch >= 2)
         hipLaunchKernelGGL((k_ragged_persistent<W, M, 2>), dim3((unsigned)g), dim3(tpw), 0, s, trk, hit, out,
-                           lane_nlay, blocks, nblocks, counter);
+                           lane_nlay, blocks, nblocks, counter, claim);
       else
         hipLaunchKernelGGL((k_ragged_persistent<W, M, 1>), dim3((unsigned)g), dim3(tpw), 0, s, trk, hit, out,
-                           lane_nlay, blocks, nblocks, counter);
+                           lane_nlay, blocks, nblocks, counter, claim);
       return hipGetLastError();
     });
   });

@@ -161,7 +161,9 @@ int launch_batches(p2z_server* sv) {
     const int64_t grid = ((int64_t)sv->ctx->cus * wpc * 64 + tpw - 1) / tpw;
     P2Z_HIP_CHECK(hipMemcpyAsync(s.d_desc, s.h_desc, nblk * sizeof(p2z_block_desc), hipMemcpyHostToDevice, s.stream));
     P2Z_HIP_CHECK(p2z::launch_ragged(sv->trk, sv->hit, sv->arena_mem, sv->lanes, s.d_desc, nblk, o.width, o.math,
-                                     true, o.prefetch ? o.prefetch : 1, tpw, grid, s.counter, s.stream));
+                                     true, o.prefetch ? o.prefetch : 2, tpw, grid,
+                                     o.items_per_claim ? o.items_per_claim : p2z::kDefaultClaim, s.counter,
+                                     s.stream));
     P2Z_HIP_CHECK(hipEventRecord(s.done, s.stream));
     const double t = now_s();
     for (int64_t id : ids) sv->reqs.at(id).t_launch = t;
@@ -193,6 +195,7 @@ int p2z_server_create(p2z_ctx* ctx, const void* trk, const void* hit, const int3
     return p2z::fail(P2Z_EINVAL, "tracks_per_wg must be 64, 128, 192 or 256");
   if (o.prefetch < 0 || o.prefetch > 2) return p2z::fail(P2Z_EINVAL, "prefetch must be 0, 1 or 2");
   if (o.waves_per_cu < 0 || o.waves_per_cu > 32) return p2z::fail(P2Z_EINVAL, "waves_per_cu in [0,32]");
+  if (o.items_per_claim < 0 || o.items_per_claim > 64) return p2z::fail(P2Z_EINVAL, "items_per_claim in [0,64]");
   const int64_t blk_bytes = (int64_t)p2z::kTrkWords * o.width * sizeof(float);
   const int64_t nblk = o.budget_bytes / blk_bytes;
   if (nblk <= 0) return p2z::fail(P2Z_EINVAL, "budget of %lld bytes holds no result block", (long long)o.budget_bytes);

@@ -76,7 +76,7 @@ BLOCK_DESC_DTYPE = np.dtype([("trk_block", "<i8"), ("hit_block", "<i8"), ("out_b
 
 class RaggedOpts(C.Structure):
     _fields_ = [("tracks_per_wg", C.c_int32), ("prefetch", C.c_int32), ("waves_per_cu", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("items_per_claim", C.c_int32)]
 
 
 class WorkloadSpec(C.Structure):
@@ -92,7 +92,7 @@ class WorkloadInfo(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("budget_bytes", C.c_int64), ("nstreams", C.c_int32), ("max_batch_blocks", C.c_int32),
                 ("width", C.c_int32), ("math", C.c_int32), ("tracks_per_wg", C.c_int32), ("prefetch", C.c_int32),
-                ("waves_per_cu", C.c_int32), ("reserved", C.c_int32)]
+                ("waves_per_cu", C.c_int32), ("items_per_claim", C.c_int32)]
 
 
 class Completion(C.Structure):
@@ -393,11 +393,11 @@ class Server:
 
     def __init__(self, ctx: "Context", trk, hit, lane_nlayer, budget_bytes: int, width: int, math="fast",
                  nstreams: int = 0, max_batch_blocks: int = 0, tracks_per_wg: int = 0, prefetch: int = 0,
-                 waves_per_cu: int = 0):
+                 waves_per_cu: int = 0, items_per_claim: int = 0):
         self.ctx = ctx
         self.width = width
         o = ServerOpts(budget_bytes, nstreams, max_batch_blocks, width, MATH.get(math, math), tracks_per_wg,
-                       prefetch, waves_per_cu, 0)
+                       prefetch, waves_per_cu, items_per_claim)
         self.handle = vp()
         check(lib().p2z_server_create(ctx.handle, _ptr(trk), _ptr(hit), _ptr(lane_nlayer), C.byref(o),
                                       C.byref(self.handle)), "p2z_server_create")
@@ -584,12 +584,12 @@ class Context:
 
     def run_ragged(self, trk, hit, out, lane_nlayer, blocks, nblocks: int, width: int, math="fast",
                    persistent: bool = True, graph: bool = False, tracks_per_wg: int = 0, prefetch: int = 0,
-                   waves_per_cu: int = 0) -> None:
+                   waves_per_cu: int = 0, items_per_claim: int = 0) -> None:
         """All blocks of a descriptor table (device buffer of BlockDesc) through
         their layers: one persistent work-queue launch (optionally replayed from
         a captured hipGraph), or one wave per 64 track slots."""
         flags = (RAGGED_PERSISTENT if persistent else 0) | (RAGGED_GRAPH if graph else 0)
-        o = RaggedOpts(tracks_per_wg, prefetch, waves_per_cu, 0)
+        o = RaggedOpts(tracks_per_wg, prefetch, waves_per_cu, items_per_claim)
         check(lib().p2z_run_ragged(self.handle, _ptr(trk), _ptr(hit), _ptr(out), _ptr(lane_nlayer), _ptr(blocks),
                                    nblocks, width, MATH.get(math, math), flags, C.byref(o)), "p2z_run_ragged")
 

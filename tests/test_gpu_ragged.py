@@ -106,10 +106,11 @@ def test_ragged_equals_plain_path_on_uniform_events(ctx):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-def test_graph_replays_equal_eager(ctx, small_ragged):
-    """Captured graph (counter reset kernel + persistent kernel) replayed
-    several times with other work in between: every replay equals the eager
-    launch."""
+@pytest.mark.parametrize("persistent", [True, False])
+def test_graph_replays_equal_eager(ctx, small_ragged, persistent):
+    """Captured graph (counter reset kernel + persistent kernel, or the
+    one-shot kernel) replayed several times with other work in between: every
+    replay equals the eager launch."""
     w = small_ragged
     d = Dev(ctx, w)
     ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast")
@@ -118,15 +119,18 @@ def test_graph_replays_equal_eager(ctx, small_ragged):
     for rep in range(4):
         ctx.run_ragged(d.trk, d.hit, scratch, d.lanes, d.blk, w.nblocks, w.width, "ref")  # other work
         ctx.h2d(d.out, np.zeros_like(eager))
-        ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast", graph=True)
+        ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast", graph=True,
+                       persistent=persistent)
         got = ctx.download(d.out)
         assert np.array_equal(got.view(np.uint32), eager.view(np.uint32)), rep
     scratch.free()
     d.free()
 
 
-@pytest.mark.parametrize("opts", [dict(prefetch=2), dict(tracks_per_wg=64), dict(waves_per_cu=2),
-                                  dict(tracks_per_wg=128, prefetch=2, waves_per_cu=16)])
+@pytest.mark.parametrize("opts", [dict(prefetch=1), dict(tracks_per_wg=64), dict(waves_per_cu=2),
+                                  dict(tracks_per_wg=128, prefetch=2, waves_per_cu=16),
+                                  dict(items_per_claim=1), dict(items_per_claim=16, waves_per_cu=1),
+                                  dict(persistent=False, prefetch=1), dict(persistent=False, tracks_per_wg=256)])
 def test_launch_shapes_are_bitwise_equal(ctx, small_ragged, opts):
     w = small_ragged
     d = Dev(ctx, w)
