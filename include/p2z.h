@@ -259,7 +259,9 @@ typedef struct {
   int32_t nstreams;         /* batches in flight at once, 1..16; 0 = 4         */
   int32_t max_batch_blocks; /* block limit of one batch; 0 = none              */
   int32_t width, math;      /* AoSoA width of the pool, arithmetic mode        */
-  int32_t tracks_per_wg, prefetch, waves_per_cu, items_per_claim; /* as p2z_ragged_opts */
+  int32_t tracks_per_wg, prefetch, waves_per_cu, items_per_claim; /* as p2z_ragged_opts, except
+                               items_per_claim 0 = 1: a batch of one event
+                               then runs one work item per wave (latency)  */
 } p2z_server_opts;
 
 typedef struct {

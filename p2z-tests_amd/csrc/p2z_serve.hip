@@ -162,7 +162,7 @@ int launch_batches(p2z_server* sv) {
     P2Z_HIP_CHECK(hipMemcpyAsync(s.d_desc, s.h_desc, nblk * sizeof(p2z_block_desc), hipMemcpyHostToDevice, s.stream));
     P2Z_HIP_CHECK(p2z::launch_ragged(sv->trk, sv->hit, sv->arena_mem, sv->lanes, s.d_desc, nblk, o.width, o.math,
                                      true, o.prefetch ? o.prefetch : 2, tpw, grid,
-                                     o.items_per_claim ? o.items_per_claim : p2z::kDefaultClaim, s.counter,
+                                     o.items_per_claim ? o.items_per_claim : 1, s.counter,
                                      s.stream));
     P2Z_HIP_CHECK(hipEventRecord(s.done, s.stream));
     const double t = now_s();
