@@ -1,7 +1,7 @@
 """Throughput of the p2z propagate-to-z + Kalman-update path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--math fast|ref|fp64gain]
-                    [--sections main,ragged,serving,precision]
+                    [--sections main,ragged,serving,ingest,precision]
 
 Headline workload (value): the reference's full configuration, 100 events x
 9600 tracks x 20 layers, AoSoA bsize 32, FP32, inputs from the reference
@@ -16,11 +16,15 @@ baseline (the reference program compiled from its own source, oracle/_ref,
 OpenMP over events, on a bounded sample of the same workload), and extra
 sections measured on synthetic workloads of the library's generator:
   ragged    : 2000 uneven events (log-normal track counts, 8-26 layers per
-              track), FP32 with the FP64 gain steps; the autotuned persistent
-              launch replayed from a hipGraph against one plain launch per event
-  serving   : the same events arriving as a Poisson stream at 1k..20k events/s
-              through the continuous-batching server (HBM budget 64 GB):
-              p50 / p99 / p999 event latency and sustained throughput
+              track), FP32 with the FP64 gain steps and FP32 alone; the
+              autotuned persistent and one-shot launches, each replayed from a
+              hipGraph, against one plain launch per event
+  serving   : the same events arriving as a Poisson stream at 1k..200k
+              events/s through the continuous-batching server (HBM budget
+              64 GB): p50 / p99 / p999 event latency and sustained throughput
+  ingest    : the headline workload from host buffers (p2z_run_host: chunked
+              copies in and out overlapping the compute on several streams),
+              pinned and pageable, beside the parts alone
   precision : 100 events x 9600 tracks x 20 layers with wide angle / momentum /
               hit-error ranges: FP32 path against the FP64-gain path
 """
@@ -224,6 +228,43 @@ def section_serving(ctx, pool: dict, rates, budget_gb: float) -> list:
     return rows
 
 
+def section_ingest(ctx, trk, hit, nevts: int, nb: int, nl: int, W: int, math: str) -> dict:
+    """The include-data mode (CU4:830-875): host buffers in, host buffer out,
+    through p2z_run_host, which copies chunks of events in, computes and copies
+    out on separate streams so that the transfers overlap each other and the
+    compute.  Timed with pinned and with pageable host memory, beside the
+    parts alone (one H2D of the inputs, one D2H of the output, the compute)."""
+    tl = nevts * nb * W * nl
+    out = np.empty_like(trk)
+    res = {"workload": f"{nevts} events x {nb * W} tracks x {nl} layers, H2D {(trk.nbytes + hit.nbytes) / 1e6:.1f} MB, "
+                       f"D2H {trk.nbytes / 1e6:.1f} MB per pass"}
+
+    def wall(fn, reps=3):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3
+
+    ms = wall(lambda: ctx.run_host(trk, hit, out, nevts, nb, nl, W, math))
+    res["pageable"] = {"ms": round(ms, 3), "tl_per_s": tl / (ms * 1e-3)}
+    pt, ph, po = ctx.pinned_copy(trk), ctx.pinned_copy(hit), ctx.pinned_empty(trk.size)
+    ms = wall(lambda: ctx.run_host(pt, ph, po, nevts, nb, nl, W, math))
+    res["pinned"] = {"ms": round(ms, 3), "tl_per_s": tl / (ms * 1e-3)}
+    dt, dh, do = ctx.alloc(trk.nbytes), ctx.alloc(hit.nbytes), ctx.alloc(trk.nbytes)
+    h2d = wall(lambda: (ctx.h2d(dt, pt), ctx.h2d(dh, ph)))
+    d2h = wall(lambda: ctx.d2h(po, do))
+    comp = time_ms(ctx, lambda: ctx.run(dt, dh, do, nevts, nb, nl, W, math), 10)
+    for b in (dt, dh, do):
+        b.free()
+    res["parts_ms"] = {"h2d_pinned": round(h2d, 3), "d2h_pinned": round(d2h, 3), "compute": round(comp, 4)}
+    res["h2d_gb_per_s"] = round((trk.nbytes + hit.nbytes) / (h2d * 1e-3) / 1e9, 1)
+    res["overlap"] = round((h2d + d2h + comp) / res["pinned"]["ms"], 3)
+    return res
+
+
 def section_precision(ctx, nevents: int) -> dict:
     """cfg 2 throughput: the FP32 path and the FP64-gain path on the wide
     (ill-conditioned) workload; the error report lives in the tests."""
@@ -251,10 +292,10 @@ def main() -> int:
     ap.add_argument("--bsize", type=int, default=32)
     ap.add_argument("--tracks-per-wg", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sections", default="main,ragged,serving,precision",
-                    help="comma list of main, ragged, serving, precision")
+    ap.add_argument("--sections", default="main,ragged,serving,ingest,precision",
+                    help="comma list of main, ragged, serving, ingest, precision")
     ap.add_argument("--ragged-events", type=int, default=2000)
-    ap.add_argument("--serving-rates", default="1000,2000,5000,10000,20000")
+    ap.add_argument("--serving-rates", default="1000,2000,5000,10000,20000,50000,200000")
     ap.add_argument("--budget-gb", type=float, default=64.0)
     a = ap.parse_args()
     sections = set(a.sections.split(","))
@@ -309,6 +350,11 @@ def main() -> int:
         for b in pool["d"].values():
             b.free()
         pool["w"].close()
+        ctx.close()
+    if "ingest" in sections:
+        ctx = p2z.Context(0)
+        log("ingest (host buffers through p2z_run_host)")
+        extra["ingest"] = section_ingest(ctx, trk, hit, nevts, nb, nl, W, a.math)
         ctx.close()
     if "precision" in sections:
         ctx = p2z.Context(0)

@@ -283,6 +283,28 @@ __device__ inline void sincos_short(float x, float* sp, float* cp, float* omcp) 
   }
 }
 
+// sin x and 1 - cos x for |x| < 2^-8 (the update's corrections of the track
+// angles are typically ~1e-4): x - x^3/6 (next term x^5/120 < 2^-46 |x|) and
+// x^2/2 (next term x^4/24 < 2^-35, absolute).  The latter enters only as a
+// product with a sine or cosine added to one, so its absolute error is what
+// counts.
+__device__ inline void sincos_tiny(float x, float* sp, float* omcp) {
+  const float u = x * x;
+  *sp = __builtin_fmaf(u * x, -0x1.555556p-3f, x);
+  *omcp = 0.5f * u;
+}
+
+// Rotation increments for a small angle: sincos_tiny below 2^-8, sincos_short
+// (sincos_unit or the library) above.
+__device__ inline void sincos_increment(float x, float* sp, float* omcp) {
+  if (__builtin_fabsf(x) < 0x1p-8f) {
+    sincos_tiny(x, sp, omcp);
+  } else {
+    float c;
+    sincos_short(x, sp, &c, omcp);
+  }
+}
+
 // The 11 Jacobian entries of one track's step.
 struct Jac {
   float J02, J03, J04, J05, J12, J13, J14, J15, J42, J43, J45;
@@ -528,10 +550,10 @@ __device__ inline void update_fast(Track& t, const Hit& h, Trig* g = nullptr) {
     const float dtheta = K0[5] * res0 + K1[5] * res1;
     p[4] = p[4] + dphi;
     p[5] = p[5] + dtheta;
-    float sd, cd, omcd;
-    sincos_short(dphi, &sd, &cd, &omcd);
+    float sd, omcd;
+    sincos_increment(dphi, &sd, &omcd);
     rotate(g->sinP, g->cosP, sd, omcd);
-    sincos_short(dtheta, &sd, &cd, &omcd);
+    sincos_increment(dtheta, &sd, &omcd);
     rotate(g->sinT, g->cosT, sd, omcd);
   }
   // P - K (H P); H P = rows 0 and 1 of P.
@@ -598,10 +620,10 @@ __device__ inline void update_fp64gain(Track& t, const Hit& h, Trig* g = nullptr
     if (i >= 4) dang[i - 4] = (float)d;
   }
   if constexpr (kCarry) {
-    float sd, cd, omcd;
-    sincos_short(dang[0], &sd, &cd, &omcd);
+    float sd, omcd;
+    sincos_increment(dang[0], &sd, &omcd);
     rotate(g->sinP, g->cosP, sd, omcd);
-    sincos_short(dang[1], &sd, &cd, &omcd);
+    sincos_increment(dang[1], &sd, &omcd);
     rotate(g->sinT, g->cosT, sd, omcd);
   }
   // Joseph form (I - K H) P (I - K H)^T + K R K^T, expanded:

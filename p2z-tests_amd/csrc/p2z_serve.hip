@@ -102,14 +102,21 @@ struct p2z_server {
 
 namespace {
 
+// Descriptor room of a slot.  Reserved up front for batches of up to
+// kSlotDescriptors blocks (hundreds of typical events) and grown geometrically
+// past that: a reallocation frees device memory, which waits for the device,
+// and allocates pinned memory, which is slow -- neither belongs in the loop.
+constexpr int64_t kSlotDescriptors = 65536;
+
 int slot_reserve(Slot& s, int64_t n) {
   if (s.cap >= n) return P2Z_OK;
+  n = std::max<int64_t>(n, 2 * s.cap);
   if (s.d_desc) P2Z_HIP_CHECK(hipFree(s.d_desc));
   if (s.h_desc) P2Z_HIP_CHECK(hipHostFree(s.h_desc));
   s.d_desc = nullptr;
   s.h_desc = nullptr;
   s.cap = 0;
-  const int64_t cap = std::max<int64_t>(n, 1024);
+  const int64_t cap = std::max<int64_t>(n, kSlotDescriptors);
   P2Z_HIP_CHECK(hipMalloc(&s.d_desc, cap * sizeof(p2z::BlockDesc)));
   P2Z_HIP_CHECK(hipHostMalloc(&s.h_desc, cap * sizeof(p2z_block_desc), hipHostMallocDefault));
   s.cap = cap;
@@ -217,6 +224,7 @@ int p2z_server_create(p2z_ctx* ctx, const void* trk, const void* hit, const int3
       if (rc == P2Z_OK) rc = p2z::hip_status(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
       if (rc == P2Z_OK) rc = p2z::hip_status(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate");
       if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc(&s.counter, 256), "hipMalloc");
+      if (rc == P2Z_OK) rc = slot_reserve(s, kSlotDescriptors);
     }
   }
   if (rc != P2Z_OK) {

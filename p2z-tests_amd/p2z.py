@@ -482,6 +482,7 @@ class Context:
         self._opts = Opts(tracks_per_wg, nstreams, hit_stage, 0)
         check(lib().p2z_create(C.byref(self.handle), device, C.byref(self._opts)), "p2z_create")
         self._bufs: list[DeviceBuffer] = []
+        self._pinned: list[int] = []
 
     # lifetime
     def close(self) -> None:
@@ -490,6 +491,9 @@ class Context:
                 if b.ptr:
                     lib().p2z_dev_free(self.handle, b.ptr)
                     b.ptr = 0
+            for p in self._pinned:
+                lib().p2z_host_free(self.handle, p)
+            self._pinned = []
             check(lib().p2z_destroy(self.handle), "p2z_destroy")
             self.handle = vp()
 
@@ -512,6 +516,20 @@ class Context:
         check(lib().p2z_dev_alloc(self.handle, C.byref(p), nbytes), "p2z_dev_alloc")
         b = DeviceBuffer(self, p.value or 0, nbytes)
         self._bufs.append(b)
+        return b
+
+    def pinned_empty(self, n: int, dtype=np.float32) -> np.ndarray:
+        """A numpy array of n elements over pinned host memory
+        (p2z_host_alloc), valid until the context is closed."""
+        nbytes = max(int(n) * np.dtype(dtype).itemsize, 1)
+        p = vp()
+        check(lib().p2z_host_alloc(self.handle, C.byref(p), nbytes), "p2z_host_alloc")
+        self._pinned.append(p.value)
+        return np.frombuffer((C.c_char * nbytes).from_address(p.value), dtype=dtype, count=int(n))
+
+    def pinned_copy(self, a: np.ndarray) -> np.ndarray:
+        b = self.pinned_empty(a.size, a.dtype)
+        b[...] = a.reshape(-1)
         return b
 
     def upload(self, a: np.ndarray) -> DeviceBuffer:
