@@ -3,7 +3,9 @@
 // and N bytes written per dispatch, so the ratio of what the counters report
 // to N converts counter readings of other kernels to bytes.
 //
-//   p2z_copy_calibrate [MiB=1024] [launches=5]
+//   p2z_copy_calibrate [MiB=1024] [launches=5] [workgroups=8192; 0 = one element per thread]
+//
+// Without a profiler it also measures the achievable copy bandwidth.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -17,6 +19,7 @@ __global__ void k_copy(const float4* __restrict__ a, float4* __restrict__ b, siz
 int main(int argc, char** argv) {
   const size_t mib = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 1024;
   const int reps = argc > 2 ? std::atoi(argv[2]) : 5;
+  const long wg = argc > 3 ? std::atol(argv[3]) : 8192;
   const size_t bytes = mib << 20, n = bytes / sizeof(float4);
   float4 *a = nullptr, *b = nullptr;
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
@@ -25,19 +28,18 @@ int main(int argc, char** argv) {
   }
   if (hipMemset(a, 1, bytes) != hipSuccess || hipMemset(b, 0, bytes) != hipSuccess) return 1;
   hipEvent_t t0, t1;
-  hipEventCreate(&t0);
-  hipEventCreate(&t1);
-  const unsigned grid = 256 * 8 * 4;
+  if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) return 1;
+  const unsigned grid = wg > 0 ? (unsigned)wg : (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
-  hipEventRecord(t0);
+  if (hipEventRecord(t0) != hipSuccess) return 2;
   for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
-  hipEventRecord(t1);
+  if (hipEventRecord(t1) != hipSuccess) return 2;
   if (hipEventSynchronize(t1) != hipSuccess) return 2;
   float ms = 0;
-  hipEventElapsedTime(&ms, t0, t1);
-  std::printf("copy %zu MiB x %d: %.4f ms per launch, %.1f GB/s (read + write)\n", mib, reps, ms / reps,
-              2.0 * bytes / (ms / reps * 1e-3) / 1e9);
-  hipFree(a);
-  hipFree(b);
+  if (hipEventElapsedTime(&ms, t0, t1) != hipSuccess) return 2;
+  std::printf("copy %zu MiB x %d, %u workgroups: %.4f ms per launch, %.1f GB/s (read + write)\n", mib, reps, grid,
+              ms / reps, 2.0 * bytes / (ms / reps * 1e-3) / 1e9);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return 0;
 }
