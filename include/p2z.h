@@ -179,7 +179,16 @@ typedef struct {
 } p2z_ragged_opts;
 
 /* Runs every block of the descriptor table (device memory) through its layers.
- * trk, hit, out, lane_nlayer, blocks: device pointers.  Asynchronous. */
+ * trk, hit, out, lane_nlayer, blocks: device pointers.  Asynchronous.
+ * The descriptors are not range-checked on the device: every trk_block,
+ * hit_block .. hit_block + nlayer - 1 and out_block must lie inside the
+ * buffers (p2z_check_blocks checks a host copy of a table). */
+/* Host-side check of a descriptor table (host memory) against the buffers it
+ * addresses: 0 <= trk_block < pool_blocks, nlayer >= 0, 0 <= hit_block and
+ * hit_block + nlayer <= pool_hit_blocks, 0 <= out_block < out_blocks.
+ * P2Z_OK, or P2Z_EINVAL naming the first descriptor that is outside. */
+int p2z_check_blocks(const p2z_block_desc* blocks, int64_t nblocks, int64_t pool_blocks,
+                     int64_t pool_hit_blocks, int64_t out_blocks);
 int p2z_run_ragged(p2z_ctx* ctx, const void* trk, const void* hit, void* out,
                    const int32_t* lane_nlayer, const p2z_block_desc* blocks, int64_t nblocks,
                    int32_t width, int32_t math, uint32_t flags, const p2z_ragged_opts* opts);
@@ -262,6 +271,9 @@ typedef struct {
   int32_t tracks_per_wg, prefetch, waves_per_cu, items_per_claim; /* as p2z_ragged_opts, except
                                items_per_claim 0 = 1: a batch of one event
                                then runs one work item per wave (latency)  */
+  int64_t pool_blocks;      /* MPTRK blocks in trk (W-lane rows of lane_nlayer) */
+  int64_t pool_hit_blocks;  /* MPHIT blocks in hit; requests are checked
+                               against both (p2z_check_blocks) on submit     */
 } p2z_server_opts;
 
 typedef struct {

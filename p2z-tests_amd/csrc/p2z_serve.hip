@@ -203,6 +203,8 @@ int p2z_server_create(p2z_ctx* ctx, const void* trk, const void* hit, const int3
   if (o.prefetch < 0 || o.prefetch > 2) return p2z::fail(P2Z_EINVAL, "prefetch must be 0, 1 or 2");
   if (o.waves_per_cu < 0 || o.waves_per_cu > 32) return p2z::fail(P2Z_EINVAL, "waves_per_cu in [0,32]");
   if (o.items_per_claim < 0 || o.items_per_claim > 64) return p2z::fail(P2Z_EINVAL, "items_per_claim in [0,64]");
+  if (o.pool_blocks <= 0 || o.pool_hit_blocks <= 0)
+    return p2z::fail(P2Z_EINVAL, "pool_blocks and pool_hit_blocks (the pool's sizes) are required");
   const int64_t blk_bytes = (int64_t)p2z::kTrkWords * o.width * sizeof(float);
   const int64_t nblk = o.budget_bytes / blk_bytes;
   if (nblk <= 0) return p2z::fail(P2Z_EINVAL, "budget of %lld bytes holds no result block", (long long)o.budget_bytes);
@@ -260,10 +262,8 @@ int p2z_server_submit(p2z_server* sv, int64_t request, const p2z_block_desc* blo
   if (nblocks > sv->arena->total())
     return p2z::fail(P2Z_EINVAL, "request %lld needs %lld result blocks, the arena holds %lld", (long long)request,
                      (long long)nblocks, (long long)sv->arena->total());
-  for (int64_t i = 0; i < nblocks; ++i)
-    if (blocks[i].out_block < 0 || blocks[i].out_block >= nblocks || blocks[i].nlayer < 0 || blocks[i].trk_block < 0 ||
-        blocks[i].hit_block < 0)
-      return p2z::fail(P2Z_EINVAL, "request %lld: bad descriptor %lld", (long long)request, (long long)i);
+  if (p2z_check_blocks(blocks, nblocks, sv->opts.pool_blocks, sv->opts.pool_hit_blocks, nblocks) != P2Z_OK)
+    return p2z::fail(P2Z_EINVAL, "request %lld: %s", (long long)request, p2z_last_error());
   const int64_t at = sv->arena->alloc(nblocks);
   if (at < 0) return p2z::fail(P2Z_EBUSY, "arena full (%lld of %lld blocks in use)", (long long)sv->arena->used(),
                                (long long)sv->arena->total());

@@ -92,7 +92,8 @@ class WorkloadInfo(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("budget_bytes", C.c_int64), ("nstreams", C.c_int32), ("max_batch_blocks", C.c_int32),
                 ("width", C.c_int32), ("math", C.c_int32), ("tracks_per_wg", C.c_int32), ("prefetch", C.c_int32),
-                ("waves_per_cu", C.c_int32), ("items_per_claim", C.c_int32)]
+                ("waves_per_cu", C.c_int32), ("items_per_claim", C.c_int32), ("pool_blocks", C.c_int64),
+                ("pool_hit_blocks", C.c_int64)]
 
 
 class Completion(C.Structure):
@@ -145,6 +146,7 @@ _SIGS = {
     "p2z_run": ([vp, vp, vp, vp, i64, i64, i32, i32, i32], i32),
     "p2z_run_host": ([vp, vp, vp, vp, i64, i64, i32, i32, i32], i32),
     "p2z_run_ragged": ([vp, vp, vp, vp, vp, vp, i64, i32, i32, u32, C.POINTER(RaggedOpts)], i32),
+    "p2z_check_blocks": ([vp, i64, i64, i64, i64], i32),
     "p2z_workload_create": ([C.POINTER(WorkloadSpec), C.POINTER(vp)], i32),
     "p2z_workload_destroy": ([vp], i32),
     "p2z_workload_get_info": ([vp, C.POINTER(WorkloadInfo)], i32),
@@ -375,6 +377,14 @@ class Workload:
 # device side
 # ---------------------------------------------------------------------------
 
+def check_blocks(blocks: np.ndarray, pool_blocks: int, pool_hit_blocks: int, out_blocks: int) -> None:
+    """p2z_check_blocks on a descriptor table (BLOCK_DESC_DTYPE array):
+    raises P2zError (EINVAL) naming the first descriptor outside the buffers."""
+    b = np.ascontiguousarray(blocks, dtype=BLOCK_DESC_DTYPE)
+    check(lib().p2z_check_blocks(b.ctypes.data if b.size else None, b.size, pool_blocks, pool_hit_blocks, out_blocks),
+          "p2z_check_blocks")
+
+
 @dataclass
 class DeviceBuffer:
     ctx: "Context"
@@ -397,7 +407,8 @@ class Server:
         self.ctx = ctx
         self.width = width
         o = ServerOpts(budget_bytes, nstreams, max_batch_blocks, width, MATH.get(math, math), tracks_per_wg,
-                       prefetch, waves_per_cu, items_per_claim)
+                       prefetch, waves_per_cu, items_per_claim, trk.nbytes // (TRK_WORDS * width * 4),
+                       hit.nbytes // (HIT_WORDS * width * 4))
         self.handle = vp()
         check(lib().p2z_server_create(ctx.handle, _ptr(trk), _ptr(hit), _ptr(lane_nlayer), C.byref(o),
                                       C.byref(self.handle)), "p2z_server_create")

@@ -121,6 +121,12 @@ def test_request_bookkeeping_errors(ctx, pool):
         bad["out_block"][0] = 10 ** 6                         # result block outside the request
         with pytest.raises(p2z.P2zError):
             sv.submit(6, bad)
+        for field, value in (("trk_block", w.nblocks), ("hit_block", w.nhit_blocks)):
+            bad = event_blocks(w, 6)
+            bad[field][-1] = value                            # a block outside the resident pool
+            with pytest.raises(p2z.P2zError) as err:
+                sv.submit(6, bad)
+            assert err.value.rc == p2z.EINVAL and "outside" in str(err.value)
         sv.wait([5])
         sv.release(5)
     with pytest.raises(p2z.P2zError):                         # budget below one block
