@@ -8,7 +8,9 @@ generator (the tracks do not depend on W, so every candidate does the same
 work).  Results are cached per device and workload kind in
 ~/.cache/p2z/autotune.json unless cache=False.
 
-    python autotune.py [--kind ragged] [--events 200]
+    python autotune.py [--kind ragged] [--events 200] [--math fp64gain]
+    python autotune.py --run FORM,W,TPW,PREFETCH[,WAVES_PER_CU,CLAIM] [--reps N]
+        (one configuration, N graph-captured launches: for profiling)
 """
 from __future__ import annotations
 
@@ -101,13 +103,36 @@ def cached_or_tune(ctx, key: str, cache: bool = True, **kw) -> dict:
     return res
 
 
+def run_one(ctx, spec: str, kind: str, nevents: int, math: str, reps: int, seed: int = 2024) -> dict:
+    """Runs one launch configuration reps times and returns its median time."""
+    f = spec.split(",")
+    form, W, tpw, pf = f[0], int(f[1]), int(f[2]), int(f[3])
+    wpc = int(f[4]) if len(f) > 4 else 0
+    cl = int(f[5]) if len(f) > 5 else 0
+    w = p2z.Workload(kind, nevents=nevents, width=W, seed=seed)
+    trk, hit = ctx.upload(w.tracks), ctx.upload(w.hits)
+    lanes = ctx.upload(np.ascontiguousarray(w.lane_nlayer))
+    blk = ctx.upload(np.ascontiguousarray(w.blocks_lpt))
+    out = ctx.alloc(w.tracks.nbytes)
+    ms = _time(ctx, lambda: ctx.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, math, persistent=form == "persistent",
+                                           graph=True, tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc,
+                                           items_per_claim=cl), reps)
+    return dict(form=form, width=W, tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc, items_per_claim=cl, ms=ms,
+                tl_per_s=w.track_layers / (ms * 1e-3), track_layers=w.track_layers, tracks=w.ntracks)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--kind", default="ragged", choices=list(p2z.WORKLOAD))
     ap.add_argument("--events", type=int, default=200)
     ap.add_argument("--math", default="fp64gain", choices=list(p2z.MATH))
+    ap.add_argument("--run", default="", help="FORM,W,TPW,PREFETCH[,WAVES_PER_CU,CLAIM]: time one configuration")
+    ap.add_argument("--reps", type=int, default=20)
     a = ap.parse_args()
     with p2z.Context(0) as ctx:
+        if a.run:
+            print(json.dumps(run_one(ctx, a.run, a.kind, a.events, a.math, a.reps)))
+            return
         res = tune(ctx, a.kind, a.events, a.math, log=print)
     print(json.dumps(res["best"]))
 
