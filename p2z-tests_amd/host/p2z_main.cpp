@@ -3,14 +3,17 @@
 //
 //   p2z_main [--preset ref|config0] [--nevts N] [--ntrks N] [--bsize W]
 //            [--nlayer L] [--niter K] [--math fast|ref|fp64gain]
-//            [--include-data] [--device-summary] [--device D]
+//            [--include-data] [--device-summary] [--per-block] [--device D]
 //
 // Defaults are the reference's (nevts 100, ntrks 9600, bsize 32, nlayer 20),
 // with NITER 10 as its Makefile sets it.  --include-data times host->device
 // and device->host copies of every pass (the reference's INCLUDE_DATA mode);
-// otherwise the timed region is the device-resident passes.  The report is
-// computed on the host exactly as the reference does, or on the GPU with
-// --device-summary.
+// otherwise the timed region is the device-resident passes.  --per-block runs
+// the reference's own loop instead (OMP:733-747: per event and block, copy the
+// track block, then per layer propagateToZ and KalmanUpdate_v2 on that block)
+// through the per-block functions of p2z.hpp -- the same results, one GPU call
+// per block and step, so for small sizes only.  The report is computed on the
+// host exactly as the reference does, or on the GPU with --device-summary.
 #include <sys/time.h>
 
 #include <chrono>
@@ -31,7 +34,25 @@ struct Args {
   int bsize = P2Z_BSIZE;
   bool include_data = false;
   bool device_summary = false;
+  bool per_block = false;
 };
+
+// The reference's driver loop (OMP:733-747) over the per-block functions.
+template <int W>
+void per_block_loop(const MPTRK_<W>* trk, const MPHIT_<W>* hit, MPTRK_<W>* outtrk) {
+  const P2zConfig& c = p2zConfig();
+  for (int64_t ie = 0; ie < c.nevts; ++ie)
+    for (int64_t ib = 0; ib < nbOf<W>(); ++ib) {
+      const MPTRK_<W>* btracks = bTk(trk, ie, ib);
+      MPTRK_<W>* obtracks = bTk(outtrk, ie, ib);
+      *obtracks = *btracks;
+      for (int layer = 0; layer < c.nlayer; ++layer) {
+        const MPHIT_<W>* bhits = bHit(hit, ie, ib, layer);
+        propagateToZ(&obtracks->cov, &obtracks->par, &obtracks->q, &bhits->pos, &obtracks->cov, &obtracks->par);
+        KalmanUpdate_v2(&obtracks->cov, &obtracks->par, &bhits->cov, &bhits->pos);
+      }
+    }
+}
 
 long now_ms() {
   struct timeval tv;
@@ -80,7 +101,12 @@ int run(const Args& a) {
   std::printf("done preparing!\n");
 
   const auto wall_start = std::chrono::high_resolution_clock::now();
-  const double gpu_s = p2zRunDriver<W>(trk, hit, outtrk, c.niter, a.include_data);
+  double gpu_s = 0;
+  if (a.per_block) {
+    for (int it = 0; it < c.niter; ++it) per_block_loop<W>(trk, hit, outtrk);
+  } else {
+    gpu_s = p2zRunDriver<W>(trk, hit, outtrk, c.niter, a.include_data);
+  }
   const auto wall_stop = std::chrono::high_resolution_clock::now();
   const double wall_time =
       std::chrono::duration_cast<std::chrono::microseconds>(wall_stop - wall_start).count() / 1e6;
@@ -90,8 +116,9 @@ int run(const Args& a) {
   std::printf("done ntracks=%lld tot time=%f (s) time/trk=%e (s)\n", ntot, wall_time, wall_time / ntot);
   std::printf("formatted %i %i %i %i %i %f 0 %f %i\n", c.niter, (int)c.nevts, (int)c.ntrks, W, (int)nb,
               wall_time, setup_s, 1);
-  std::printf("gpu time=%f (s) time/trk=%e (s) track-layer updates/s=%e\n", gpu_s, gpu_s / ntot,
-              (double)ntot * c.nlayer / gpu_s);
+  if (!a.per_block)
+    std::printf("gpu time=%f (s) time/trk=%e (s) track-layer updates/s=%e\n", gpu_s, gpu_s / ntot,
+                (double)ntot * c.nlayer / gpu_s);
 
   const float last[3] = {inputhits[c.nlayer - 1].pos[0], inputhits[c.nlayer - 1].pos[1],
                          inputhits[c.nlayer - 1].pos[2]};
@@ -144,6 +171,7 @@ int main(int argc, char** argv) {
       if (c.math < 0) usage("math must be fast, ref or fp64gain");
     } else if (k == "--include-data") a.include_data = true;
     else if (k == "--device-summary") a.device_summary = true;
+    else if (k == "--per-block") a.per_block = true;
     else usage(("unknown option " + k).c_str());
   }
   if (c.nevts <= 0 || c.ntrks <= 0 || c.nlayer <= 0 || c.niter <= 0) usage("sizes must be positive");

@@ -395,3 +395,21 @@ def test_dropin_program_report(golden):
         s = O.summary(O.run(trk, hit, nevts, nb, 32, 20), nevts, nb, 32, ntrks, last)
         expect = O.format_summary(s).splitlines()
     assert rep == expect
+
+
+def test_dropin_program_per_block_loop():
+    """bin/p2z_main --per-block: the reference's own loop (OMP:733-747) through
+    the per-block propagateToZ / KalmanUpdate_v2 of the C++ mirror gives the
+    same report, to the last digit, as the fused pass (reference typing; both
+    paths evaluate each step the same way)."""
+    args = [p2z.MAIN_PATH, "--nevts", "2", "--ntrks", "320", "--niter", "1", "--math", "ref"]
+
+    def report(extra):
+        res = subprocess.run(args + extra, capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stderr
+        return [ln for ln in res.stdout.splitlines()
+                if (ln.startswith("track ") and not ln.startswith("track in")) or ln.startswith("number of")]
+
+    fused, per_block = report([]), report(["--per-block"])
+    log("\n".join(per_block))
+    assert len(fused) == 11 and per_block == fused
