@@ -352,4 +352,20 @@ int p2z_workload_load(const char* path, p2z_workload** out) {
   return P2Z_OK;
 }
 
+int p2z_check_blocks(const p2z_block_desc* blocks, int64_t nblocks, int64_t pool_blocks, int64_t pool_hit_blocks,
+                     int64_t out_blocks) {
+  if (nblocks < 0 || (nblocks > 0 && !blocks)) return p2z::fail(P2Z_EINVAL, "null or negative descriptor table");
+  for (int64_t i = 0; i < nblocks; ++i) {
+    const p2z_block_desc& d = blocks[i];
+    if (d.trk_block < 0 || d.trk_block >= pool_blocks || d.nlayer < 0 || d.hit_block < 0 ||
+        d.hit_block > pool_hit_blocks - d.nlayer || d.out_block < 0 || d.out_block >= out_blocks)
+      return p2z::fail(P2Z_EINVAL,
+                       "descriptor %lld outside the buffers (trk_block %lld of %lld, hit blocks %lld+%d of %lld, "
+                       "out_block %lld of %lld)",
+                       (long long)i, (long long)d.trk_block, (long long)pool_blocks, (long long)d.hit_block,
+                       d.nlayer, (long long)pool_hit_blocks, (long long)d.out_block, (long long)out_blocks);
+  }
+  return P2Z_OK;
+}
+
 }  // extern "C"
