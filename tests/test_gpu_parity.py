@@ -413,3 +413,43 @@ def test_dropin_program_per_block_loop():
     fused, per_block = report([]), report(["--per-block"])
     log("\n".join(per_block))
     assert len(fused) == 11 and per_block == fused
+
+
+def test_rare_branches_on_the_precision_workload(ctx):
+    """The wide-range precision workload (helix-consistent hits) through the
+    fused pass takes the production modes' rarely used branches, checked here
+    on the host: turning angles above 1 rad (reduced-argument sin/cos) and
+    update corrections of phi above 2^-8 and above 1 rad (the larger rotation
+    increments of the carried sin/cos).  On it: REF is bitwise the oracle
+    (correctly rounded sin/cos); FAST and FP64GAIN are at least as accurate
+    against the FP64 oracle as the reference FP32 algorithm (per-element
+    medians), and FP64GAIN improves the covariance."""
+    W, nl = 32, 20
+    w = p2z.Workload("precision", nevents=2, width=W, seed=21, ntracks=2048, nlayer=nl)
+    nb = w.nblocks
+    trk, hit = np.array(w.tracks), np.array(w.hits)
+    t = trk.reshape(nb, 28, W).astype(np.float64)
+    h0 = hit.reshape(nb, nl, 9, W)[:, 0, 2, :].astype(np.float64)
+    q = trk.reshape(nb, 28, W)[:, 27, :].view(np.int32).astype(np.float64)
+    k = q * (100.0 / (-0.299792458 * 3.8112))
+    alpha = (h0 - t[:, 2]) * np.sin(t[:, 5]) * t[:, 3] / (np.cos(t[:, 5]) * k)
+    first = O.run_fp64(trk, np.ascontiguousarray(hit.reshape(nb, nl, 9, W)[:, :1]).reshape(-1), 1, nb, W, 1)
+    dphi = first[:, 4] - (t[:, 4] + alpha).reshape(-1)
+    cover = (np.mean(np.abs(alpha) > 1), np.mean(np.abs(dphi) > 2.0 ** -8), np.mean(np.abs(dphi) > 1))
+    log(f"  |alpha| > 1: {cover[0]:.3f}; |dphi| > 2^-8: {cover[1]:.3f}; |dphi| > 1: {cover[2]:.4f}")
+    assert cover[0] > 0.01 and cover[1] > 0.2 and cover[2] > 0.002
+    nev, nbe = 2, nb // 2
+    o64 = O.run_fp64(trk, hit, nev, nbe, W, nl)
+    base = compare(track_major(O.run(trk, hit, nev, nbe, W, nl), W), o64)
+    with O.use_variant("cr"):
+        ref_oracle = O.run(trk, hit, nev, nbe, W, nl)
+    assert words_equal(gpu_run(ctx, trk, hit, nev, nbe, nl, W, "ref"), ref_oracle) == 1.0
+    # beyond the median this workload is chaotic for the FP32 algorithm itself
+    # (the reference's p99 parameter error against FP64 is O(10)): medians only
+    res = {}
+    for math in ("fast", "fp64gain"):
+        st = res[math] = compare(track_major(gpu_run(ctx, trk, hit, nev, nbe, nl, W, math), W), o64)
+        log(f"  {math} vs fp64: {st}\n  reference vs fp64: {base}")
+        assert st["nonfinite_mismatch"] <= base["nonfinite_mismatch"] and st["charge_mismatch"] == 0
+        assert st["par_median"] <= 1.1 * base["par_median"] and st["cov_median"] <= 1.1 * base["cov_median"], math
+    assert res["fp64gain"]["cov_median"] <= res["fast"]["cov_median"]
