@@ -139,7 +139,8 @@ def ragged_bytes(w) -> float:
 
 
 def section_ragged(ctx, nevents: int, sample_events: int) -> tuple[dict, dict]:
-    """cfg 3: autotune (per arithmetic mode; by default on the event set
+    """Ragged events (2000 events, log-normal track counts, 8-26 layers):
+    autotune (per arithmetic mode; by default on the event set
     itself: a persistent grid needs several work chunks per wave before its
     timing carries over, which a small sample does not give), then the full
     event set through the persistent and the one-shot launch, each replayed
@@ -209,7 +210,8 @@ def section_ragged(ctx, nevents: int, sample_events: int) -> tuple[dict, dict]:
 
 
 def section_serving(ctx, pool: dict, rates, budget_gb: float) -> list:
-    """cfg 4: Poisson arrivals of the resident events through the server."""
+    """Serving: Poisson arrivals of the resident ragged events through the
+    continuous-batching server."""
     w, d, W, opts = pool["w"], pool["d"], pool["W"], dict(pool["opts"])
     # latency first: one work item per wave and two layers of hits in flight
     opts.update(items_per_claim=1, prefetch=2)
@@ -266,7 +268,7 @@ def section_ingest(ctx, trk, hit, nevts: int, nb: int, nl: int, W: int, math: st
 
 
 def section_precision(ctx, nevents: int) -> dict:
-    """cfg 2 throughput: the FP32 path and the FP64-gain path on the wide
+    """Precision workload throughput: the FP32 path and the FP64-gain path on the wide
     (ill-conditioned) workload; the error report lives in the tests."""
     w = p2z.Workload("precision", nevents=nevents, width=32, seed=42, ntracks=9600, nlayer=20)
     trk, hit, out = ctx.upload(w.tracks), ctx.upload(w.hits), ctx.alloc(w.tracks.nbytes)

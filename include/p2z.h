@@ -41,7 +41,7 @@ extern "C" {
 #define P2Z_EBUSY (-5)    /* serving: no free slot / budget right now       */
 #define P2Z_ESTATE (-6)   /* call not valid in the object's current state   */
 
-/* arithmetic modes (see DESIGN.md of the repository for the numerical contract) */
+/* arithmetic modes */
 #define P2Z_MATH_FAST 0     /* FP32, FMA contraction, sparse forms (default)  */
 #define P2Z_MATH_REF 1      /* the reference CPU typing: FP64 sub-expressions,
                                no contraction, reference summation order      */
