@@ -177,13 +177,18 @@ __device__ __forceinline__ void ragged_item(const float* __restrict__ trk, const
   if (valid) store_track<W>(t, out + d.out_block * (kTrkWords * W) + l);
 }
 
-// Persistent form: each wave takes chunks of `claim` consecutive items from a
-// device counter until it is handed a chunk past the end (every wave reaches
-// that exit).  A wave claims its next chunk before computing the current one,
-// so the atomic's round trip overlaps the computation; every chunk is claimed
-// exactly once, so every item is computed exactly once.  Several items per
-// claim keep the single counter's atomics (serialised where they meet) few.
-// The counter starts at 0 (k_reset_counter runs first on the same stream).
+// Persistent form: the items are cut into chunks of `claim` consecutive items.
+// Wave w of the grid (nwaves in all) computes chunk w first, with no atomic:
+// a counter that every wave hit at start would serialise thousands of atomics
+// before the first item is computed.  After that a wave takes chunk
+// nwaves + atomicAdd(counter, 1) until it is handed a chunk past the end
+// (every wave reaches that exit).  The atomic is issued when the last item of
+// the current chunk starts, so its round trip overlaps that item; a wave
+// never holds a whole chunk in reserve while others run dry at the end.
+// Every chunk index is handed out once, so every item is computed exactly
+// once.  Several items per claim keep the single counter's atomics
+// (serialised where they meet) few.  The counter starts at 0
+// (k_reset_counter runs first on the same stream).
 template <int W, int kMath, int kPrefetch>
 __global__ __launch_bounds__(256) void k_ragged_persistent(const float* __restrict__ trk,
                                                            const float* __restrict__ hit,
@@ -194,16 +199,17 @@ __global__ __launch_bounds__(256) void k_ragged_persistent(const float* __restri
                                                            int claim) {
   const int64_t nitems = (nblocks * W + 63) / 64;
   const int lane = threadIdx.x & 63;
-  unsigned mine = 0;
-  if (lane == 0) mine = atomicAdd(counter, 1u);
-  int64_t first = (int64_t)(unsigned)__shfl((int)mine, 0) * claim;
-  while (first < nitems) {
-    unsigned next = 0;
-    if (lane == 0) next = atomicAdd(counter, 1u);
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / 64);
+  int64_t chunk = (int64_t)blockIdx.x * (blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  while (chunk * claim < nitems) {
+    const int64_t first = chunk * claim;
     const int64_t last = first + claim < nitems ? first + claim : nitems;
-    for (int64_t item = first; item < last; ++item)
+    unsigned next = 0;
+    for (int64_t item = first; item < last; ++item) {
+      if (item == last - 1 && lane == 0) next = atomicAdd(counter, 1u);
       ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, lane);
-    first = (int64_t)(unsigned)__shfl((int)next, 0) * claim;
+    }
+    chunk = nwaves + (int64_t)(unsigned)__shfl((int)next, 0);
   }
 }
 
