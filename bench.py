@@ -123,8 +123,11 @@ def cpu_baseline_port(nevts: int, ntrks: int, nlayer: int) -> dict | None:
 
 
 def time_ms(ctx, fn, reps: int) -> float:
-    """Average hipEvent time of fn() over reps calls (after one warm call)."""
-    fn()
+    """Average hipEvent time of fn() over reps back-to-back calls, after reps
+    untimed calls: the GPU idles while the host generates a workload, and
+    its clocks take a few launches to come back up."""
+    for _ in range(reps):
+        fn()
     ctx.sync()
     ctx.timer_start()
     for _ in range(reps):
@@ -150,41 +153,49 @@ def section_ragged(ctx, nevents: int, sample_events: int) -> tuple[dict, dict]:
 
     res = {"math_modes": {}}
     pools = {}
-    for math in ("fp64gain", "fast"):
-        log(f"ragged: autotuning {math} on {sample_events} events")
-        tuned = autotune.tune(ctx, "ragged", nevents=sample_events, math=math, reps=3, seed=2024, log=log)
-        best = tuned["best"]
-        W = best["width"]
-        at_w = lambda form: min((r for r in tuned["results"] if r["form"] == form and r["width"] == W),
-                                key=lambda r: r["ms"])
-        bp, bo = at_w("persistent"), at_w("oneshot")
-        opts = dict(tracks_per_wg=bp["tracks_per_wg"], prefetch=bp["prefetch"], waves_per_cu=bp["waves_per_cu"],
-                    items_per_claim=bp["items_per_claim"])
-        oopts = dict(tracks_per_wg=bo["tracks_per_wg"], prefetch=bo["prefetch"])
-        log(f"ragged: {math} best {best}; generating {nevents} events at width {W}")
+
+    def pool(W):
         if W not in pools:
+            log(f"ragged: generating {nevents} events at width {W}")
             w = p2z.Workload("ragged", nevents=nevents, width=W, seed=2024)
             pools[W] = dict(w=w, d=dict(trk=ctx.upload(w.tracks), hit=ctx.upload(w.hits),
                                         lanes=ctx.upload(np.ascontiguousarray(w.lane_nlayer)),
                                         lpt=ctx.upload(np.ascontiguousarray(w.blocks_lpt)),
                                         evt=ctx.upload(np.ascontiguousarray(w.blocks))))
-        w, d = pools[W]["w"], pools[W]["d"]
-        nbytes = ragged_bytes(w)
+        return pools[W]["w"], pools[W]["d"]
+
+    for math in ("fp64gain", "fast"):
+        log(f"ragged: autotuning {math} on {sample_events} events")
+        tuned = autotune.tune(ctx, "ragged", nevents=sample_events, math=math, reps=3, seed=2024, log=log)
+        best = tuned["best"]
+        # each launch form at its own best width
+        bp, bo = tuned["best_persistent"], tuned["best_oneshot"]
+        opts = dict(tracks_per_wg=bp["tracks_per_wg"], prefetch=bp["prefetch"], waves_per_cu=bp["waves_per_cu"],
+                    items_per_claim=bp["items_per_claim"])
+        oopts = dict(tracks_per_wg=bo["tracks_per_wg"], prefetch=bo["prefetch"])
+        log(f"ragged: {math} best {best}")
+        row = {"tuned": {"form": best["form"], "persistent": dict(width=bp["width"], **opts),
+                         "oneshot": dict(width=bo["width"], **oopts),
+                         "candidates": tuned["candidates"], "sample_events": sample_events}}
+
+        def rate(w, ms):
+            return {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3),
+                    "hbm_frac": round(ragged_bytes(w) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+
+        W = bp["width"]
+        w, d = pool(W)
         res["workload"] = (f"{nevents} events, {w.ntracks} tracks, {w.track_layers} track-layers "
                            f"(tracks/event log-normal median 2000 sigma 1 in [50,20000], layers U{{8..26}})")
         out = ctx.alloc(w.tracks.nbytes)
-        row = {"tuned": {"form": best["form"], "width": W, "persistent": opts, "oneshot": oopts,
-                         "candidates": tuned["candidates"], "sample_events": sample_events}}
-
-        def rate(ms):
-            return {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3),
-                    "hbm_frac": round(nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
-
-        row["persistent_graph"] = rate(time_ms(ctx, lambda: ctx.run_ragged(
-            d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, math, graph=True, **opts), 5))
-        row["oneshot_graph"] = rate(time_ms(ctx, lambda: ctx.run_ragged(
+        row["persistent_graph"] = dict(width=W, **rate(w, time_ms(ctx, lambda: ctx.run_ragged(
+            d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, math, graph=True, **opts), 20)))
+        out.free()
+        W = bo["width"]
+        w, d = pool(W)
+        out = ctx.alloc(w.tracks.nbytes)
+        row["oneshot_graph"] = dict(width=W, **rate(w, time_ms(ctx, lambda: ctx.run_ragged(
             d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, math, persistent=False, graph=True,
-            **oopts), 5))
+            **oopts), 20)))
         eb = w.event_blocks
 
         def per_event():
@@ -193,19 +204,19 @@ def section_ragged(ctx, nevents: int, sample_events: int) -> tuple[dict, dict]:
                 ctx.run_ragged(d["trk"], d["hit"], out, d["lanes"], d["evt"].ptr + b0 * 32, b1 - b0, W, math,
                                persistent=False, **oopts)
 
-        row["one_launch_per_event"] = {**rate(time_ms(ctx, per_event, 1)), "launches": nevents}
+        row["one_launch_per_event"] = {**rate(w, time_ms(ctx, per_event, 1)), "width": W, "launches": nevents}
         best_row = max((row["persistent_graph"], row["oneshot_graph"]), key=lambda r: r["tl_per_s"])
         row["speedup_vs_per_event"] = round(best_row["tl_per_s"] / row["one_launch_per_event"]["tl_per_s"], 2)
         out.free()
         res["math_modes"][math] = row
     res["bytes_per_pass"] = "24 B x track-layers + 224 B x tracks (algorithmic)"
-    W0 = res["math_modes"]["fp64gain"]["tuned"]["width"]
-    opts0 = res["math_modes"]["fp64gain"]["tuned"]["persistent"]
-    for W, pool in pools.items():
+    opts0 = dict(res["math_modes"]["fp64gain"]["tuned"]["persistent"])
+    W0 = opts0.pop("width")
+    for W, pl in pools.items():
         if W != W0:
-            for b in pool["d"].values():
+            for b in pl["d"].values():
                 b.free()
-            pool["w"].close()
+            pl["w"].close()
     return res, dict(w=pools[W0]["w"], d=pools[W0]["d"], W=W0, opts=opts0)
 
 

@@ -1,9 +1,11 @@
-"""Autotuner of the persistent ragged launch.
+"""Autotuner of the ragged launch (persistent work queue or one-shot).
 
 Sweeps the launch form (persistent work queue or one-shot), the AoSoA width
 W in {16, 32, 64, 128}, tracks per workgroup, hit prefetch depth and, for the
-persistent form, the grid size (waves per CU) on a sample of the workload, timing each candidate with hipEvents (median of repeats), and
-returns the fastest.  The sample is generated at each width by the same
+persistent form, the grid size (waves per CU) and the items per claim on a
+sample of the workload, timing each candidate with hipEvents (sustained time
+per launch over back-to-back launches, median of three rounds), and returns
+the fastest.  The sample is generated at each width by the same
 generator (the tracks do not depend on W, so every candidate does the same
 work).  Results are cached per device and workload kind in
 ~/.cache/p2z/autotune.json unless cache=False.
@@ -33,14 +35,22 @@ CLAIMS = (2, 4, 8)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "p2z", "autotune.json")
 
 
-def _time(ctx, fn, reps: int) -> float:
-    fn()
+def _time(ctx, fn, reps: int, rounds: int = 3) -> float:
+    """Sustained time per launch: the median over `rounds` of the hipEvent
+    time of `reps` back-to-back launches, divided by reps.  Back-to-back
+    launches keep the GPU under load, as in a production pass; single
+    launches after an idle gap run at higher clocks and rank the candidates
+    differently.  reps untimed launches come first: after the GPU idles (the
+    host generating a sample), its clocks take a few launches to settle."""
+    for _ in range(reps):
+        fn()
     ctx.sync()
     ts = []
-    for _ in range(reps):
+    for _ in range(rounds):
         ctx.timer_start()
-        fn()
-        ts.append(ctx.timer_stop())
+        for _ in range(reps):
+            fn()
+        ts.append(ctx.timer_stop() / reps)
     return float(np.median(ts))
 
 
@@ -104,7 +114,8 @@ def cached_or_tune(ctx, key: str, cache: bool = True, **kw) -> dict:
 
 
 def run_one(ctx, spec: str, kind: str, nevents: int, math: str, reps: int, seed: int = 2024) -> dict:
-    """Runs one launch configuration reps times and returns its median time."""
+    """Times one launch configuration (graph-captured; sustained time per
+    launch over reps back-to-back launches, median of three rounds)."""
     f = spec.split(",")
     form, W, tpw, pf = f[0], int(f[1]), int(f[2]), int(f[3])
     wpc = int(f[4]) if len(f) > 4 else 0
