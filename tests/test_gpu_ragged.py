@@ -106,12 +106,18 @@ def test_ragged_equals_plain_path_on_uniform_events(ctx):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-@pytest.mark.parametrize("persistent", [True, False])
-def test_graph_replays_equal_eager(ctx, small_ragged, persistent):
+# A persistent grid smaller than the work: beyond each wave's first chunk the
+# items come from the work-queue counter, so the counter reset matters.
+SMALL_GRID = dict(tracks_per_wg=64, waves_per_cu=1, items_per_claim=1)
+
+
+@pytest.mark.parametrize("persistent,opts", [(True, {}), (True, SMALL_GRID), (False, {})])
+def test_graph_replays_equal_eager(ctx, small_ragged, persistent, opts):
     """Captured graph (counter reset kernel + persistent kernel, or the
     one-shot kernel) replayed several times with other work in between: every
     replay equals the eager launch."""
     w = small_ragged
+    assert (w.nblocks * w.width + 63) // 64 > 256  # more items than the small grid has waves (256 CUs)
     d = Dev(ctx, w)
     ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast")
     eager = ctx.download(d.out)
@@ -120,7 +126,7 @@ def test_graph_replays_equal_eager(ctx, small_ragged, persistent):
         ctx.run_ragged(d.trk, d.hit, scratch, d.lanes, d.blk, w.nblocks, w.width, "ref")  # other work
         ctx.h2d(d.out, np.zeros_like(eager))
         ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast", graph=True,
-                       persistent=persistent)
+                       persistent=persistent, **opts)
         got = ctx.download(d.out)
         assert np.array_equal(got.view(np.uint32), eager.view(np.uint32)), rep
     scratch.free()
@@ -130,6 +136,8 @@ def test_graph_replays_equal_eager(ctx, small_ragged, persistent):
 @pytest.mark.parametrize("opts", [dict(prefetch=1), dict(tracks_per_wg=64), dict(waves_per_cu=2),
                                   dict(tracks_per_wg=128, prefetch=2, waves_per_cu=16),
                                   dict(items_per_claim=1), dict(items_per_claim=16, waves_per_cu=1),
+                                  SMALL_GRID, dict(SMALL_GRID, items_per_claim=3),
+                                  dict(SMALL_GRID, items_per_claim=7, prefetch=1),
                                   dict(persistent=False, prefetch=1), dict(persistent=False, tracks_per_wg=256)])
 def test_launch_shapes_are_bitwise_equal(ctx, small_ragged, opts):
     w = small_ragged
