@@ -174,8 +174,9 @@ typedef struct {
   int32_t prefetch;      /* hit layers loaded ahead: 1 or 2; 0 = 2             */
   int32_t waves_per_cu;  /* persistent grid = CUs * waves_per_cu waves; 0 = 8  */
   int32_t items_per_claim; /* persistent form: work items (64 track slots
-                              each) a wave takes per atomic on the queue
-                              counter, 1..64; 0 = 4                          */
+                              each) per chunk, 1..64; 0 = 4.  Wave w of the
+                              grid computes chunk w first, then takes one
+                              chunk per atomic on the queue counter       */
 } p2z_ragged_opts;
 
 /* Runs every block of the descriptor table (device memory) through its layers.
