@@ -11,7 +11,9 @@ is one pass of the reference driver loop (src/propagate-toz-test_OMP.cpp:
 resident in HBM.
 
 One JSON line: value = track-layer updates/s, a roofline object for the fused
-kernel (algorithmic HBM bytes per launch / hipEvent launch time), the CPU
+kernel (algorithmic HBM bytes per launch / hipEvent launch time; its `valu`
+member is the VALU issue fraction from the lane-instructions per track-layer
+that rocprofv3 counted on this workload), the CPU
 baseline (the reference program compiled from its own source, oracle/_ref,
 OpenMP over events, on a bounded sample of the same workload), and extra
 sections measured on synthetic workloads of the library's generator:
@@ -46,7 +48,10 @@ import numpy as np  # noqa: E402
 import p2z  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (non-packed lane-instructions/s)
+VALU_PEAK = 256 * 4 * 16 * 2.4e9
 PROFILE_TRAFFIC = os.path.join(HERE, "profiles", "traffic_fused_w32_100ev.txt")
+PROFILE_VALU = os.path.join(HERE, "profiles", "valu_counters_fused_w32_100ev.md")
 
 
 def log(msg: str) -> None:
@@ -60,18 +65,35 @@ def bytes_per_track_layer(nlayer: int) -> float:
     return 24.0 + 224.0 / nlayer
 
 
+def _profile_value(path: str, key: str, math: str):
+    """The figure `key[math] = value` of a committed profile summary, if any."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            m = re.match(rf"{key}\[{math}\]\s*=\s*([0-9.eE+]+)", ln.strip())
+            if m:
+                return float(m.group(1))
+    return None
+
+
 def read_traffic(math: str):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration), if the committed
     profile summary for this workload and math mode exists."""
-    if not os.path.exists(PROFILE_TRAFFIC):
+    return _profile_value(PROFILE_TRAFFIC, "traffic_bytes_per_launch", math)
+
+
+def valu_roofline(math: str, tl_per_s: float):
+    """VALU issue fraction: lane-instructions per track-layer counted by
+    rocprofv3 --pmc SQ_INSTS_VALU on this workload (committed profile), times
+    the measured rate, against the VALU issue peak."""
+    per_tl = _profile_value(PROFILE_VALU, "valu_lane_instr_per_tl", math)
+    if per_tl is None:
         return None
-    with open(PROFILE_TRAFFIC) as f:
-        for ln in f:
-            m = re.match(rf"traffic_bytes_per_launch\[{math}\]\s*=\s*([0-9.eE+]+)", ln.strip())
-            if m:
-                return float(m.group(1))
-    return None
+    rate = per_tl * tl_per_s
+    return {"lane_instr_per_tl": per_tl, "achieved": rate, "peak": VALU_PEAK, "unit": "lane-instr/s",
+            "frac": round(rate / VALU_PEAK, 4)}
 
 
 def cpu_baseline(nevts: int, ntrks: int, nlayer: int) -> dict | None:
@@ -346,9 +368,12 @@ def main() -> int:
 
     b_tl = bytes_per_track_layer(nl)
     achieved = tl_per_step * b_tl / (ms_per_step * 1e-3) / 1e9
+    default_workload = (nevts, ntrks, nl, W) == (100, 9600, 20, 32)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(a.math),
-                "bytes_per_track_layer": b_tl, "launch_ms": round(ms_per_step, 5)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(a.math) if default_workload else None,
+                "bytes_per_track_layer": b_tl, "launch_ms": round(ms_per_step, 5),
+                "valu": valu_roofline(a.math, value) if default_workload else None}
 
     extra = {}
     if "ragged" in sections or "serving" in sections:
