@@ -142,7 +142,9 @@ int p2z_run(p2z_ctx* ctx, const void* trk, const void* hit, void* out, int64_t n
 /* Same with host pointers (the reference's include-data mode, CU4:830-875):
  * chunks of events are copied in, computed and copied out on the context's
  * streams so that transfers overlap compute.  Returns when out is complete.
- * Pinned buffers (p2z_host_alloc) give full PCIe rate. */
+ * Pinned buffers (p2z_host_alloc) give full PCIe rate; from a pinned hit
+ * buffer only the 6 words of each record the path reads (pos, cov xx xy yy)
+ * cross PCIe, not the unused cov xz yz zz (OMP:510-512). */
 int p2z_run_host(p2z_ctx* ctx, const void* trk, const void* hit, void* out, int64_t nevts,
                  int64_t nb, int32_t nlayer, int32_t width, int32_t math);
 

@@ -287,6 +287,21 @@ int p2z_run(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_t nev
   return P2Z_OK;
 }
 
+namespace {
+
+// True for page-locked host memory (hipHostMalloc); pageable memory makes
+// hipPointerGetAttributes fail or report unregistered memory.
+bool host_pinned(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
+}  // namespace
+
 int p2z_run_host(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_t nevts, int64_t nb,
                  int32_t nlayer, int32_t width, int32_t math) {
   if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
@@ -294,6 +309,7 @@ int p2z_run_host(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_
   P2Z_TRY(use_device(c));
   if (nevts == 0 || nb == 0) return P2Z_OK;
   const int ns = (int)c->pipe.size();
+  const bool hit_pinned = nlayer > 0 && host_pinned(hit);
   // Chunks of whole events, two per stream so that copies of one chunk overlap
   // the compute of the previous one on the other streams.
   const int64_t nchunks = std::min<int64_t>(nevts, 2 * (int64_t)ns);
@@ -326,8 +342,20 @@ int p2z_run_host(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_
     const int64_t ne = e1 - e0;
     P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_trk[i], (const char*)trk + e0 * trk_ev, ne * trk_ev,
                                  hipMemcpyHostToDevice, s));
-    P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_hit[i], (const char*)hit + e0 * hit_ev, ne * hit_ev,
-                                 hipMemcpyHostToDevice, s));
+    if (hit_pinned) {
+      // only the words the path reads: pos x, y, z and cov xx, xy, yy of each
+      // hit record (the record's last 3W words, cov xz, yz, zz, are never
+      // read, OMP:510-512); one row per record, same pitch on both sides
+      const size_t pitch = (size_t)p2z::kHitWords * width * sizeof(float);
+      P2Z_HIP_CHECK(hipMemcpy2DAsync(c->slot_hit[i], pitch, (const char*)hit + e0 * hit_ev, pitch,
+                                     (size_t)p2z::kHitUsedWords * width * sizeof(float),
+                                     (size_t)(ne * nb * nlayer), hipMemcpyHostToDevice, s));
+    } else {
+      // pageable memory: one contiguous copy (a strided copy from pageable
+      // memory goes row by row through the runtime's staging)
+      P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_hit[i], (const char*)hit + e0 * hit_ev, ne * hit_ev,
+                                   hipMemcpyHostToDevice, s));
+    }
     P2Z_HIP_CHECK(p2z::launch_fused((const float*)c->slot_trk[i], (const float*)c->slot_hit[i],
                                     (float*)c->slot_out[i], ne * nb, nlayer, width, math, c->cfg, s));
     P2Z_HIP_CHECK(hipMemcpyAsync((char*)out + e0 * trk_ev, c->slot_out[i], ne * trk_ev,

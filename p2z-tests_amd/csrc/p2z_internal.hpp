@@ -9,6 +9,9 @@ namespace p2z {
 // Words per AoSoA block of width W (reference MPTRK / MPHIT, OMP:115-125).
 constexpr int kTrkWords = 28;
 constexpr int kHitWords = 9;
+// Leading words of a hit record the path reads: pos x, y, z and cov xx, xy,
+// yy (KalmanUpdate_v2 uses the 2x2 block only, OMP:510-512).
+constexpr int kHitUsedWords = 6;
 
 inline bool valid_width(int w) { return w == 16 || w == 32 || w == 64 || w == 128; }
 inline bool valid_math(int m) { return m >= 0 && m <= 2; }

@@ -331,6 +331,22 @@ def test_host_pipeline_equals_device_run(ctx):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+@pytest.mark.parametrize("math", ["ref", "fast", "fp64gain"])
+def test_host_pipeline_pinned_copies_only_used_hit_words(ctx, math):
+    """From pinned buffers the pipeline copies only the 6 hit words per record
+    the path reads; the other 3 (cov xz, yz, zz) are never read, so filling
+    them with NaN on the host changes nothing.  Same bits as the device run."""
+    nevts, ntrks, W, nl = 7, 1024, 32, 20
+    trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
+    a = gpu_run(ctx, trk, hit, nevts, nb, nl, W, math)
+    ptrk, phit = ctx.pinned_copy(trk), ctx.pinned_copy(hit)
+    phit.reshape(-1, 9, W)[:, 6:, :] = np.nan
+    pout = ctx.pinned_empty(trk.size)
+    log(f"case run_host/pinned/{math}")
+    ctx.run_host(ptrk, phit, pout, nevts, nb, nl, W, math)
+    assert np.array_equal(a.view(np.uint32), pout.view(np.uint32))
+
+
 def test_deterministic_across_runs_and_launch_shapes(ctx):
     nevts, ntrks, W, nl = 3, 4096, 32, 20
     trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
