@@ -32,6 +32,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "p2z_cov.hpp"
+
 namespace p2z {
 
 enum Math : int { kFast = 0, kRef = 1, kFp64Gain = 2 };
@@ -40,10 +42,6 @@ enum Math : int { kFast = 0, kRef = 1, kFp64Gain = 2 };
 // expression (OMP:622).
 constexpr float kKfact = 100. / (-0.299792458 * 3.8112);
 
-// Packed lower-triangular index of (i, j) (reference SymOffsets66, OMP:52-55).
-__host__ __device__ constexpr int S(int i, int j) {
-  return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
-}
 
 struct Track {
   float p[6];   // x y z ipt phi theta
@@ -306,9 +304,6 @@ __device__ inline void sincos_increment(float x, float* sp, float* omcp) {
 }
 
 // The 11 Jacobian entries of one track's step.
-struct Jac {
-  float J02, J03, J04, J05, J12, J13, J14, J15, J42, J43, J45;
-};
 
 // sin and cos of a track's phi and theta.  The fused layer loops carry them
 // from layer to layer, rotating them by each change of the angles (alpha in
@@ -432,10 +427,15 @@ __device__ inline Jac helix_fast(Track& t, float hz) {
   return helix_core<kFirst>(t, hz, g);
 }
 
+
 // P' = Jt P Jt^T.  kZRow0: the input z row/column is known to be zero.
 template <bool kZRow0>
 __device__ inline void propagate_cov_fast(float* P, const Jac& J) {
 #pragma clang fp contract(fast)
+  if constexpr (kZRow0) {
+    propagate_cov_fast_z0(P, J);
+    return;
+  }
   // rows 0, 1, 4 of T = Jt P (rows 3 and 5 are rows of P; row 2 is zero).
   // Column 2 of T is only needed when P's z column can be non-zero, and then
   // only for the (i, j) sums below through T[i][2] * J.j2.
