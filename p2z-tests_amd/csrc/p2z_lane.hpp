@@ -369,8 +369,8 @@ __device__ inline Jac helix_core(Track& t, float hz, Trig& g) {
   const float ipt = p[3];
   const float pt = fast_rcp(ipt);
   const float icosT = fast_rcp(cosT);
-  const float icosTk = icosT * rk;
-  const float dzk = deltaZ * icosTk;
+  const float dzc = deltaZ * icosT;
+  const float dzk = dzc * rk;
   const float J43 = sinT * dzk;
   const float alpha = J43 * ipt;
   float sina, cosa, omca;
@@ -392,7 +392,7 @@ __device__ inline Jac helix_core(Track& t, float hz, Trig& g) {
   const float B = __builtin_fmaf(-sinP, sC, 1.0f);
   const float cc = cosP * cosP;
   const float C = __builtin_fmaf(cc, sC, sinP);
-  const float E = deltaZ * cosa * icosT;
+  const float E = dzc * cosa;
   const float F = E * icosT;
   const float G = E * sinT * pt;
   const float kpp = kpt * pt;
@@ -411,7 +411,7 @@ __device__ inline Jac helix_core(Track& t, float hz, Trig& g) {
     const float sca = sinT * cosa * icosT;
     J.J02 = -cB * sca;
     J.J12 = (cc * sC - sinP) * sca;
-    J.J42 = -ipt * sinT * icosTk;
+    J.J42 = -ipt * sinT * (icosT * rk);
   } else {
     J.J02 = J.J12 = J.J42 = 0.f;
   }
