@@ -18,8 +18,15 @@ inline bool valid_math(int m) { return m >= 0 && m <= 2; }
 
 struct LaunchCfg {
   int tracks_per_wg = 256;  // threads per workgroup, one track per thread
-  int hit_stage = 2;        // layers of hits in flight ahead of the computed one (1 or 2)
+  int hit_stage = 0;        // layers of hits in flight ahead of the computed one (1 or 2);
+                            // 0: per arithmetic mode (default_hit_stage)
 };
+
+// Hit-staging depth of the fused pass when none is set: FAST runs 5 waves per
+// SIMD at depth 1 (94 VGPRs) against 4 at depth 2 (103) and measures about 1%
+// faster; the FP64-gain and reference modes run at 3-4 waves either way and
+// keep depth 2.
+inline int default_hit_stage(int math) { return math == 0 ? 1 : 2; }
 
 // Fused driver pass: ntracks = nblocks*W tracks, all with nlayer layers.
 hipError_t launch_fused(const float* trk, const float* hit, float* out, int64_t nblocks,

@@ -486,7 +486,8 @@ hipError_t launch_fused(const float* trk, const float* hit, float* out, int64_t 
   return dispatch_width(width, [&](auto wc) {
     return dispatch_math(math, [&](auto mc) {
       constexpr int W = decltype(wc)::value, M = decltype(mc)::value;
-      if (cfg.hit_stage == 1)
+      const int depth = cfg.hit_stage ? cfg.hit_stage : default_hit_stage(math);
+      if (depth == 1)
         hipLaunchKernelGGL((k_fused<W, M, 1>), dim3((unsigned)grid_for(ntracks, tpw)), dim3(tpw), 0, s,
                            trk, hit, out, ntracks, nlayer);
       else
