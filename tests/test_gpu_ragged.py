@@ -177,3 +177,43 @@ def test_accuracy_against_fp64(ctx, kind):
     assert f["cov_median"] <= 2 * base["cov_median"] + 1e-9
     assert g["par_median"] <= base["par_median"] * 1.01 + 1e-9
     assert g["cov_median"] <= base["cov_median"]
+
+
+def test_full_size_forms_agree(ctx):
+    """The 2000-event ragged workload at full size (about 1.1e8 track-layers):
+    the persistent launch (default shape and a grid smaller than the work),
+    the one-shot launch and one launch per event give the same bits, since
+    each lane's arithmetic does not depend on the launch; every live track
+    ends on its last hit's z plane."""
+    W = 32
+    w = p2z.Workload("ragged", nevents=2000, width=W, seed=2024)
+    log(f"case ragged/full-size tracks={w.ntracks} track-layers={w.track_layers}")
+    d = Dev(ctx, w)
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, W, "fast")
+    ref = ctx.download(d.out)
+    for opts in (dict(graph=True), dict(SMALL_GRID, items_per_claim=8), dict(persistent=False, graph=True)):
+        ctx.h2d(d.out, np.zeros_like(ref))
+        ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, W, "fast", **opts)
+        got = ctx.download(d.out)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), opts
+    # one launch per event, over the event-ordered descriptors
+    evt = ctx.upload(np.ascontiguousarray(w.blocks))
+    ctx.h2d(d.out, np.zeros_like(ref))
+    eb = w.event_blocks
+    for e in range(w.nevents):
+        b0, b1 = int(eb[e]), int(eb[e + 1])
+        ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, evt.ptr + b0 * 32, b1 - b0, W, "fast", persistent=False)
+    got = ctx.download(d.out)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    evt.free()
+    # z on the last hit plane, finite parameters, for every live lane
+    t = ref.reshape(w.nblocks, 28, W)
+    hits = np.asarray(w.hits).reshape(-1, 9, W)
+    n = np.asarray(w.lane_nlayer).reshape(w.nblocks, W)
+    hb = np.asarray(w.blocks)["hit_block"]
+    live = n > 0
+    bi, li = np.nonzero(live)
+    zlast = hits[hb[bi] + n[bi, li] - 1, 2, li]
+    assert np.array_equal(t[bi, 2, li], zlast)
+    assert np.all(np.isfinite(t[bi, :6, li]))
+    d.free()
