@@ -155,3 +155,36 @@ def test_poisson_report(ctx, pool, rate):
     assert 0 < rep.lat_p50 <= rep.lat_p99 <= rep.lat_p999 <= rep.lat_max
     assert np.all(lat > 0)
     assert abs(np.median(lat) - rep.lat_p50) <= 1e-3 + 0.5 * rep.lat_p50
+
+
+def test_full_size_served_equals_batch():
+    """The serving workload at full size: 2000 ragged events with the FP64
+    gain steps, arriving at 200k events/s so that many share batches and
+    overlap on the streams.  Every served event equals the same event in one
+    ragged batch launch, bit for bit."""
+    with p2z.Context(0) as c:
+        w = p2z.Workload("ragged", nevents=2000, width=32, seed=2024)
+        W = w.width
+        log(f"case serve/full-size events={w.nevents} tracks={w.ntracks}")
+        trk, hit = c.upload(w.tracks), c.upload(w.hits)
+        lanes = c.upload(np.ascontiguousarray(w.lane_nlayer))
+        blk = c.upload(np.ascontiguousarray(w.blocks))
+        out = c.alloc(w.tracks.nbytes)
+        c.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, "fp64gain")
+        ref = c.download(out)
+        out.free()
+        blk.free()
+        with p2z.Server(c, trk, hit, lanes, 64 * 2**30, W, "fp64gain", nstreams=4) as sv:
+            rep, lat, ob = sv.poisson(w.blocks, w.event_blocks, rate=200000.0, seed=5, release=False)
+            log(f"  {rep.batches} batches, {rep.mean_batch_events:.2f} events/batch, "
+                f"max in flight {rep.max_inflight_events}")
+            assert rep.nevents == w.nevents and rep.batches < w.nevents
+            nb = np.diff(w.event_blocks)
+            for e in range(w.nevents):
+                got = sv.result(p2z.Completion(e, int(ob[e]), int(nb[e]), 0, 0, 0, 0))
+                b0, b1 = int(w.event_blocks[e]), int(w.event_blocks[e + 1])
+                assert np.array_equal(got.view(np.uint32), ref[b0 * 28 * W: b1 * 28 * W].view(np.uint32)), e
+            for e in range(w.nevents):
+                sv.release(e)
+        for b in (trk, hit, lanes):
+            b.free()
