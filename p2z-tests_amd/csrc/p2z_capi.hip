@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -90,14 +91,15 @@ int p2z_device_count(int* n) {
   return P2Z_OK;
 }
 
-int p2z_create(p2z_ctx** out, int device, const p2z_opts* opts) {
+static int create_impl(p2z_ctx** out, int device, const p2z_opts* opts) {
   if (!out) return p2z::fail(P2Z_EINVAL, "null ctx pointer");
   *out = nullptr;
   int n = 0;
   const hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess || device < 0 || device >= n)
     return p2z::fail(P2Z_ENODEV, "no GPU %d (%d visible)", device, e == hipSuccess ? n : 0);
-  p2z_ctx* c = new p2z_ctx;
+  std::unique_ptr<p2z_ctx, int (*)(p2z_ctx*)> owner(new p2z_ctx, &p2z_destroy);
+  p2z_ctx* c = owner.get();
   c->device = device;
   int rc = use_device(c);
   if (rc == P2Z_OK) rc = p2z::hip_status(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -109,12 +111,13 @@ int p2z_create(p2z_ctx** out, int device, const p2z_opts* opts) {
   if (rc == P2Z_OK)
     rc = p2z::hip_status(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device),
                          "hipDeviceGetAttribute");
-  if (rc != P2Z_OK) {
-    p2z_destroy(c);
-    return rc;
-  }
-  *out = c;
+  if (rc != P2Z_OK) return rc;
+  *out = owner.release();
   return P2Z_OK;
+}
+
+int p2z_create(p2z_ctx** out, int device, const p2z_opts* opts) {
+  return p2z::guarded("p2z_create", [&] { return create_impl(out, device, opts); });
 }
 
 int p2z_destroy(p2z_ctx* c) {
@@ -137,12 +140,16 @@ int p2z_destroy(p2z_ctx* c) {
   return P2Z_OK;
 }
 
-int p2z_set_opts(p2z_ctx* c, const p2z_opts* o) {
+static int set_opts_impl(p2z_ctx* c, const p2z_opts* o) {
   if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
   P2Z_TRY(use_device(c));
   P2Z_HIP_CHECK(hipStreamSynchronize(c->stream));
   for (auto s : c->pipe) P2Z_HIP_CHECK(hipStreamSynchronize(s));
   return apply_opts(c, o);
+}
+
+int p2z_set_opts(p2z_ctx* c, const p2z_opts* o) {
+  return p2z::guarded("p2z_set_opts", [&] { return set_opts_impl(c, o); });
 }
 
 int p2z_sync(p2z_ctx* c) {
@@ -302,8 +309,8 @@ bool host_pinned(const void* p) {
 
 }  // namespace
 
-int p2z_run_host(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_t nevts, int64_t nb,
-                 int32_t nlayer, int32_t width, int32_t math) {
+static int run_host_impl(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_t nevts, int64_t nb,
+                         int32_t nlayer, int32_t width, int32_t math) {
   if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
   P2Z_TRY(check_blocks(trk, hit, out, nevts, nb, nlayer, width, math));
   P2Z_TRY(use_device(c));
@@ -365,9 +372,15 @@ int p2z_run_host(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_
   return P2Z_OK;
 }
 
-int p2z_run_ragged(p2z_ctx* c, const void* trk, const void* hit, void* out, const int32_t* lane_nlayer,
-                   const p2z_block_desc* blocks, int64_t nblocks, int32_t width, int32_t math, uint32_t flags,
-                   const p2z_ragged_opts* opts) {
+int p2z_run_host(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_t nevts, int64_t nb,
+                 int32_t nlayer, int32_t width, int32_t math) {
+  return p2z::guarded("p2z_run_host",
+                      [&] { return run_host_impl(c, trk, hit, out, nevts, nb, nlayer, width, math); });
+}
+
+static int run_ragged_impl(p2z_ctx* c, const void* trk, const void* hit, void* out, const int32_t* lane_nlayer,
+                           const p2z_block_desc* blocks, int64_t nblocks, int32_t width, int32_t math, uint32_t flags,
+                           const p2z_ragged_opts* opts) {
   if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
   if (!trk || !hit || !out || !lane_nlayer || !blocks) return p2z::fail(P2Z_EINVAL, "null buffer");
   if (!p2z::valid_width(width)) return p2z::fail(P2Z_EINVAL, "width must be 16/32/64/128 (got %d)", width);
@@ -429,6 +442,14 @@ int p2z_run_ragged(p2z_ctx* c, const void* trk, const void* hit, void* out, cons
   }
   P2Z_HIP_CHECK(hipGraphLaunch(it->second, c->stream));
   return P2Z_OK;
+}
+
+int p2z_run_ragged(p2z_ctx* c, const void* trk, const void* hit, void* out, const int32_t* lane_nlayer,
+                   const p2z_block_desc* blocks, int64_t nblocks, int32_t width, int32_t math, uint32_t flags,
+                   const p2z_ragged_opts* opts) {
+  return p2z::guarded("p2z_run_ragged", [&] {
+    return run_ragged_impl(c, trk, hit, out, lane_nlayer, blocks, nblocks, width, math, flags, opts);
+  });
 }
 
 int p2z_summary_device(p2z_ctx* c, const void* out, int64_t nevts, int64_t nb, int32_t width,

@@ -2,12 +2,30 @@
 // returns a P2Z_* status; the text of the last failure is kept per thread and
 // returned by p2z_last_error().
 #pragma once
+#include <exception>
+#include <new>
+
 #include "p2z.h"
 
 namespace p2z {
 
 // Records a message for p2z_last_error() and returns code.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+// Runs body() and turns a C++ exception (a failed host allocation in the
+// standard containers) into a status, so that none crosses the C ABI.
+template <class F>
+int guarded(const char* what, F&& body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fail(P2Z_ENOMEM, "%s: host allocation failed", what);
+  } catch (const std::exception& e) {
+    return fail(P2Z_EINVAL, "%s: %s", what, e.what());
+  } catch (...) {
+    return fail(P2Z_EINVAL, "%s: unexpected exception", what);
+  }
+}
 
 }  // namespace p2z
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <deque>
 #include <map>
+#include <memory>
 #include <random>
 #include <unordered_map>
 #include <vector>
@@ -188,8 +189,8 @@ extern "C" {
 
 double p2z_now(void) { return now_s(); }
 
-int p2z_server_create(p2z_ctx* ctx, const void* trk, const void* hit, const int32_t* lane_nlayer,
-                      const p2z_server_opts* opts, p2z_server** out) {
+static int server_create_impl(p2z_ctx* ctx, const void* trk, const void* hit, const int32_t* lane_nlayer,
+                              const p2z_server_opts* opts, p2z_server** out) {
   if (!ctx || !trk || !hit || !lane_nlayer || !opts || !out) return p2z::fail(P2Z_EINVAL, "null argument");
   *out = nullptr;
   p2z_server_opts o = *opts;
@@ -209,7 +210,8 @@ int p2z_server_create(p2z_ctx* ctx, const void* trk, const void* hit, const int3
   const int64_t nblk = o.budget_bytes / blk_bytes;
   if (nblk <= 0) return p2z::fail(P2Z_EINVAL, "budget of %lld bytes holds no result block", (long long)o.budget_bytes);
   P2Z_TRY(p2z::use_device(ctx));
-  auto* sv = new p2z_server;
+  std::unique_ptr<p2z_server, int (*)(p2z_server*)> owner(new p2z_server, &p2z_server_destroy);
+  p2z_server* sv = owner.get();
   sv->ctx = ctx;
   sv->opts = o;
   sv->trk = static_cast<const float*>(trk);
@@ -229,12 +231,15 @@ int p2z_server_create(p2z_ctx* ctx, const void* trk, const void* hit, const int3
       if (rc == P2Z_OK) rc = slot_reserve(s, kSlotDescriptors);
     }
   }
-  if (rc != P2Z_OK) {
-    p2z_server_destroy(sv);
-    return rc;
-  }
-  *out = sv;
+  if (rc != P2Z_OK) return rc;
+  *out = owner.release();
   return P2Z_OK;
+}
+
+int p2z_server_create(p2z_ctx* ctx, const void* trk, const void* hit, const int32_t* lane_nlayer,
+                      const p2z_server_opts* opts, p2z_server** out) {
+  return p2z::guarded("p2z_server_create",
+                      [&] { return server_create_impl(ctx, trk, hit, lane_nlayer, opts, out); });
 }
 
 int p2z_server_destroy(p2z_server* sv) {
@@ -254,8 +259,8 @@ int p2z_server_destroy(p2z_server* sv) {
   return P2Z_OK;
 }
 
-int p2z_server_submit(p2z_server* sv, int64_t request, const p2z_block_desc* blocks, int64_t nblocks,
-                      double t_arrival) {
+static int server_submit_impl(p2z_server* sv, int64_t request, const p2z_block_desc* blocks, int64_t nblocks,
+                              double t_arrival) {
   if (!sv || (!blocks && nblocks)) return p2z::fail(P2Z_EINVAL, "null argument");
   if (nblocks <= 0) return p2z::fail(P2Z_EINVAL, "a request needs at least one block");
   if (sv->reqs.count(request)) return p2z::fail(P2Z_EINVAL, "request %lld already submitted", (long long)request);
@@ -279,7 +284,13 @@ int p2z_server_submit(p2z_server* sv, int64_t request, const p2z_block_desc* blo
   return P2Z_OK;
 }
 
-int p2z_server_pump(p2z_server* sv) {
+int p2z_server_submit(p2z_server* sv, int64_t request, const p2z_block_desc* blocks, int64_t nblocks,
+                      double t_arrival) {
+  return p2z::guarded("p2z_server_submit",
+                      [&] { return server_submit_impl(sv, request, blocks, nblocks, t_arrival); });
+}
+
+static int server_pump_impl(p2z_server* sv) {
   if (!sv) return p2z::fail(P2Z_EINVAL, "null server");
   P2Z_TRY(p2z::use_device(sv->ctx));
   P2Z_TRY(harvest(sv));
@@ -287,12 +298,20 @@ int p2z_server_pump(p2z_server* sv) {
   return (int)sv->ready.size();
 }
 
-int p2z_server_poll(p2z_server* sv, p2z_completion* out, int32_t cap) {
+int p2z_server_pump(p2z_server* sv) {
+  return p2z::guarded("p2z_server_pump", [&] { return server_pump_impl(sv); });
+}
+
+static int server_poll_impl(p2z_server* sv, p2z_completion* out, int32_t cap) {
   if (!sv || (!out && cap > 0)) return p2z::fail(P2Z_EINVAL, "null argument");
   const int n = std::min<int>(cap, (int)sv->ready.size());
   for (int i = 0; i < n; ++i) out[i] = sv->ready[i];
   sv->ready.erase(sv->ready.begin(), sv->ready.begin() + n);
   return n;
+}
+
+int p2z_server_poll(p2z_server* sv, p2z_completion* out, int32_t cap) {
+  return p2z::guarded("p2z_server_poll", [&] { return server_poll_impl(sv, out, cap); });
 }
 
 int p2z_server_release(p2z_server* sv, int64_t request) {
@@ -307,9 +326,9 @@ int p2z_server_release(p2z_server* sv, int64_t request) {
 
 const void* p2z_server_arena(const p2z_server* sv) { return sv ? sv->arena_mem : nullptr; }
 
-int p2z_serve_poisson(p2z_server* sv, const p2z_block_desc* blocks, const int64_t* event_blocks, int64_t nevents,
-                      double rate, uint64_t seed, int32_t release, p2z_serve_report* rep, double* latency,
-                      int64_t* out_block) {
+static int serve_poisson_impl(p2z_server* sv, const p2z_block_desc* blocks, const int64_t* event_blocks,
+                              int64_t nevents, double rate, uint64_t seed, int32_t release, p2z_serve_report* rep,
+                              double* latency, int64_t* out_block) {
   if (!sv || !blocks || !event_blocks || !rep) return p2z::fail(P2Z_EINVAL, "null argument");
   if (nevents <= 0 || !(rate > 0)) return p2z::fail(P2Z_EINVAL, "need nevents > 0 and rate > 0");
   if (!sv->reqs.empty()) return p2z::fail(P2Z_ESTATE, "the server must be idle");
@@ -392,6 +411,14 @@ int p2z_serve_poisson(p2z_server* sv, const p2z_block_desc* blocks, const int64_
   if (latency)
     for (int64_t e = 0; e < nevents; ++e) latency[e] = lat[e];
   return P2Z_OK;
+}
+
+int p2z_serve_poisson(p2z_server* sv, const p2z_block_desc* blocks, const int64_t* event_blocks, int64_t nevents,
+                      double rate, uint64_t seed, int32_t release, p2z_serve_report* rep, double* latency,
+                      int64_t* out_block) {
+  return p2z::guarded("p2z_serve_poisson", [&] {
+    return serve_poisson_impl(sv, blocks, event_blocks, nevents, rate, seed, release, rep, latency, out_block);
+  });
 }
 
 }  // extern "C"

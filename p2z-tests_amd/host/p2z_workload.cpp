@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -32,6 +33,9 @@ struct p2z_workload {
 };
 
 namespace {
+
+// A file closed when it goes out of scope.
+using File = std::unique_ptr<FILE, int (*)(FILE*)>;
 
 constexpr int kTrkWords = 28, kHitWords = 9;
 constexpr double kKfact64 = 100. / (-0.299792458 * 3.8112);
@@ -152,11 +156,21 @@ int build(const p2z_workload_spec& sp, p2z_workload& w) {
   const int nthr = sp.nthreads > 0 ? sp.nthreads : omp_get_max_threads();
   // phase 1: sizes and offsets
   std::vector<std::vector<int>> lays(ne);
+  // an exception may not leave a parallel region: a failed allocation is
+  // flagged here and thrown after the loop
+  int alloc_failed = 0;
 #pragma omp parallel for num_threads(nthr) schedule(dynamic, 8)
   for (int64_t e = 0; e < ne; ++e) {
-    lays[e] = event_layers(sp, e);
+    try {
+      lays[e] = event_layers(sp, e);
+    } catch (const std::bad_alloc&) {
+#pragma omp atomic write
+      alloc_failed = 1;
+      continue;
+    }
     std::stable_sort(lays[e].begin(), lays[e].end(), std::greater<int>());
   }
+  if (alloc_failed) throw std::bad_alloc();
   w.event_blocks.assign(ne + 1, 0);
   w.event_tracks.assign(ne, 0);
   std::vector<int64_t> hit_off(ne + 1, 0);
@@ -239,7 +253,7 @@ bool field_u(const std::string& h, const char* key, unsigned long long* v) {
 
 extern "C" {
 
-int p2z_workload_create(const p2z_workload_spec* sp, p2z_workload** out) {
+static int workload_create_impl(const p2z_workload_spec* sp, p2z_workload** out) {
   if (!sp || !out) return p2z::fail(P2Z_EINVAL, "null argument");
   *out = nullptr;
   if (sp->kind != P2Z_WORKLOAD_PRECISION && sp->kind != P2Z_WORKLOAD_RAGGED)
@@ -248,15 +262,16 @@ int p2z_workload_create(const p2z_workload_spec* sp, p2z_workload** out) {
   if (sp->nevents < 0 || sp->nevents > (1 << 24)) return p2z::fail(P2Z_EINVAL, "bad event count");
   if (sp->kind == P2Z_WORKLOAD_PRECISION && (sp->ntracks <= 0 || sp->nlayer <= 0 || sp->nlayer > 1000))
     return p2z::fail(P2Z_EINVAL, "precision workload needs ntracks > 0 and 0 < nlayer <= 1000");
-  auto* w = new p2z_workload;
+  std::unique_ptr<p2z_workload> w(new p2z_workload);
   w->spec = *sp;
   const int rc = build(*sp, *w);
-  if (rc != P2Z_OK) {
-    delete w;
-    return rc;
-  }
-  *out = w;
+  if (rc != P2Z_OK) return rc;
+  *out = w.release();
   return P2Z_OK;
+}
+
+int p2z_workload_create(const p2z_workload_spec* sp, p2z_workload** out) {
+  return p2z::guarded("p2z_workload_create", [&] { return workload_create_impl(sp, out); });
 }
 
 int p2z_workload_destroy(p2z_workload* w) {
@@ -278,7 +293,7 @@ const p2z_block_desc* p2z_workload_blocks_lpt(const p2z_workload* w) { return w 
 const int64_t* p2z_workload_event_blocks(const p2z_workload* w) { return w ? w->event_blocks.data() : nullptr; }
 const int32_t* p2z_workload_event_tracks(const p2z_workload* w) { return w ? w->event_tracks.data() : nullptr; }
 
-int p2z_workload_save(const p2z_workload* w, const char* path) {
+static int workload_save_impl(const p2z_workload* w, const char* path) {
   if (!w || !path) return p2z::fail(P2Z_EINVAL, "null argument");
   FILE* f = fopen(path, "wb");
   if (!f) return p2z::fail(P2Z_EINVAL, "cannot open %s for writing", path);
@@ -308,10 +323,15 @@ int p2z_workload_save(const p2z_workload* w, const char* path) {
   return ok ? P2Z_OK : p2z::fail(P2Z_EINVAL, "write error on %s", path);
 }
 
-int p2z_workload_load(const char* path, p2z_workload** out) {
+int p2z_workload_save(const p2z_workload* w, const char* path) {
+  return p2z::guarded("p2z_workload_save", [&] { return workload_save_impl(w, path); });
+}
+
+static int workload_load_impl(const char* path, p2z_workload** out) {
   if (!path || !out) return p2z::fail(P2Z_EINVAL, "null argument");
   *out = nullptr;
-  FILE* f = fopen(path, "rb");
+  const File file(fopen(path, "rb"), &fclose);
+  FILE* f = file.get();
   if (!f) return p2z::fail(P2Z_EINVAL, "cannot open %s", path);
   std::string hdr;
   for (int c; (c = fgetc(f)) != EOF && c != '\n';) hdr.push_back((char)c);
@@ -323,18 +343,15 @@ int p2z_workload_load(const char* path, p2z_workload** out) {
             field(hdr, "spec_ntracks", &snt) && field(hdr, "spec_nlayer", &snl) && field(hdr, "ntracks", &nt) &&
             field(hdr, "nblocks", &nb) && field(hdr, "nhit_blocks", &nh) && field(hdr, "track_layers", &tl);
   for (int k = 0; ok && k < 7; ++k) ok = field_u(hdr, sums[k], &c[k]);
-  if (!ok || !valid_width((int)width) || ne < 0 || nb < 0 || nh < 0) {
-    fclose(f);
+  if (!ok || !valid_width((int)width) || ne < 0 || nb < 0 || nh < 0)
     return p2z::fail(P2Z_EINVAL, "%s: not a p2z workload file", path);
-  }
-  auto* w = new p2z_workload;
+  std::unique_ptr<p2z_workload> w(new p2z_workload);
   w->spec = p2z_workload_spec{(int32_t)kind, (int32_t)width, ne, seed, (int32_t)snt, (int32_t)snl, 0, 0};
   w->info = p2z_workload_info{ne, nt, nb, nh, tl, (int32_t)width, (int32_t)kind};
   const size_t W = (size_t)width;
   ok = get(f, w->trk, (size_t)nb * kTrkWords * W) && get(f, w->hit, (size_t)nh * kHitWords * W) &&
        get(f, w->lane_nlay, (size_t)nb * W) && get(f, w->blocks, (size_t)nb) && get(f, w->blocks_lpt, (size_t)nb) &&
        get(f, w->event_blocks, (size_t)ne + 1) && get(f, w->event_tracks, (size_t)ne);
-  fclose(f);
   if (ok) {
     const uint64_t got[7] = {fnv1a(w->trk.data(), w->trk.size() * 4), fnv1a(w->hit.data(), w->hit.size() * 4),
                              fnv1a(w->lane_nlay.data(), w->lane_nlay.size() * 4),
@@ -344,12 +361,13 @@ int p2z_workload_load(const char* path, p2z_workload** out) {
                              fnv1a(w->event_tracks.data(), w->event_tracks.size() * 4)};
     for (int k = 0; k < 7; ++k) ok = ok && got[k] == c[k];
   }
-  if (!ok) {
-    delete w;
-    return p2z::fail(P2Z_EINVAL, "%s: truncated or corrupt (checksum mismatch)", path);
-  }
-  *out = w;
+  if (!ok) return p2z::fail(P2Z_EINVAL, "%s: truncated or corrupt (checksum mismatch)", path);
+  *out = w.release();
   return P2Z_OK;
+}
+
+int p2z_workload_load(const char* path, p2z_workload** out) {
+  return p2z::guarded("p2z_workload_load", [&] { return workload_load_impl(path, out); });
 }
 
 int p2z_check_blocks(const p2z_block_desc* blocks, int64_t nblocks, int64_t pool_blocks, int64_t pool_hit_blocks,

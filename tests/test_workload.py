@@ -157,6 +157,28 @@ def test_save_load_round_trip(ragged, tmp_path):
         p2z.Workload.load(str(tmp_path / "missing.p2zw"))
 
 
+def test_load_of_an_impossible_header_fails_cleanly(ragged, tmp_path):
+    """A header that claims more blocks than host memory holds: the loader's
+    failed allocation comes back as a status (no exception crosses the C ABI,
+    no crash), and the library keeps working."""
+    path = str(tmp_path / "events.p2zw")
+    ragged.save(path)
+    with open(path, "rb") as f:
+        hdr, rest = f.read().split(b"\n", 1)
+    big = hdr.replace(b'"nblocks":%d' % ragged.nblocks, b'"nblocks":%d' % (1 << 40))
+    assert big != hdr
+    bad = str(tmp_path / "huge.p2zw")
+    with open(bad, "wb") as f:
+        f.write(big + b"\n" + rest)
+    with pytest.raises(p2z.P2zError) as e:
+        p2z.Workload.load(bad)
+    assert e.value.rc in (p2z.ENOMEM, p2z.EINVAL)
+    assert "p2z_workload_load" in str(e.value) or "allocation" in str(e.value)
+    w = p2z.Workload.load(path)
+    assert w.nblocks == ragged.nblocks
+    w.close()
+
+
 def test_check_blocks_rejects_outside_descriptors(ragged):
     w = ragged
     b = np.array(w.blocks[:8])
