@@ -1,0 +1,113 @@
+// Issue rates of the VALU instructions the layer step is made of, the
+// calibration point for the VALU side of the roofline: what a kernel that
+// does nothing but one instruction sustains on this GPU, against the
+// 4-cycles-per-wave64-instruction peak at the reported maximum engine clock.
+//
+//   p2z_valu_rates [workgroups per CU = 8] [iterations = 4096]
+//
+// Each thread runs `iterations` rounds of 8 independent chains of one
+// operation (8 workgroups of 256 threads per CU: 8 waves per SIMD), and
+// stores one value so that nothing is optimised away.  Each operation is
+// timed over 20 launches (hipEvents) after 3 untimed ones, all after about
+// 0.2 s of warm-up.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+namespace {
+
+enum Op { kFmaF32, kMulF32, kRcpF32, kFmaF64, kMulF64, kCvtMulCvt, kRcpF64, kNumOps };
+const char* const kNames[kNumOps] = {"v_fma_f32", "v_mul_f32", "v_rcp_f32", "v_fma_f64", "v_mul_f64",
+                                     "cvt_f64_f32 + v_mul_f64 + cvt_f32_f64", "v_rcp_f64"};
+
+template <int kOp>
+__global__ __launch_bounds__(256) void k_rate(float* out, float a, float b, int iters) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  float x[8];
+  double d[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    x[i] = a + tid * 1e-7f + i;
+    d[i] = x[i];
+  }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if constexpr (kOp == kFmaF32) x[i] = __builtin_fmaf(x[i], a, b);
+      if constexpr (kOp == kMulF32) x[i] = x[i] * a;
+      if constexpr (kOp == kRcpF32) x[i] = __builtin_amdgcn_rcpf(x[i]);
+      if constexpr (kOp == kFmaF64) d[i] = __builtin_fma(d[i], (double)a, (double)b);
+      if constexpr (kOp == kMulF64) d[i] = d[i] * (double)a;
+      if constexpr (kOp == kCvtMulCvt) {
+        d[i] = d[i] * (double)x[i];
+        x[i] = (float)d[i];
+      }
+      if constexpr (kOp == kRcpF64) d[i] = __builtin_amdgcn_rcp(d[i]);
+    }
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) s += x[i] + (float)d[i];
+  out[tid] = s;
+}
+
+template <int kOp>
+int time_op(float* out, unsigned grid, int iters, float* ms) {
+  hipEvent_t t0, t1;
+  if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) return 2;
+  for (int r = 0; r < 3; ++r) hipLaunchKernelGGL(k_rate<kOp>, dim3(grid), dim3(256), 0, 0, out, 0.999f, 1e-3f, iters);
+  if (hipEventRecord(t0) != hipSuccess) return 2;
+  for (int r = 0; r < 20; ++r) hipLaunchKernelGGL(k_rate<kOp>, dim3(grid), dim3(256), 0, 0, out, 0.999f, 1e-3f, iters);
+  if (hipEventRecord(t1) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) return 2;
+  if (hipEventElapsedTime(ms, t0, t1) != hipSuccess) return 2;
+  *ms /= 20;
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const int wg_per_cu = argc > 1 ? std::atoi(argv[1]) : 8;
+  const int iters = argc > 2 ? std::atoi(argv[2]) : 4096;
+  if (wg_per_cu < 1 || wg_per_cu > 64 || iters < 1) {
+    std::fprintf(stderr, "usage: p2z_valu_rates [workgroups per CU 1..64] [iterations >= 1]\n");
+    return 1;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 1;
+  const unsigned grid = (unsigned)(prop.multiProcessorCount * wg_per_cu);
+  float* out = nullptr;
+  if (hipMalloc(&out, (size_t)grid * 256 * sizeof(float)) != hipSuccess) return 1;
+  // one wave64 instruction per SIMD every 4 cycles, 4 SIMDs per CU
+  const double peak = prop.multiProcessorCount * 4.0 * (prop.clockRate * 1e3) / 4.0;
+  const double wave_instr = (double)grid * 4 * iters * 8;  // of the measured operation, per launch
+  float ms[kNumOps];
+  // the engine clock ramps up under load: about 0.2 s of FMA work first
+  int rc = time_op<kFmaF32>(out, grid, iters, &ms[kFmaF32]);
+  for (int r = 0; r < 8 && rc == 0; ++r) rc |= time_op<kFmaF32>(out, grid, iters, &ms[kFmaF32]);
+  rc |= time_op<kFmaF32>(out, grid, iters, &ms[kFmaF32]);
+  rc |= time_op<kMulF32>(out, grid, iters, &ms[kMulF32]);
+  rc |= time_op<kRcpF32>(out, grid, iters, &ms[kRcpF32]);
+  rc |= time_op<kFmaF64>(out, grid, iters, &ms[kFmaF64]);
+  rc |= time_op<kMulF64>(out, grid, iters, &ms[kMulF64]);
+  rc |= time_op<kCvtMulCvt>(out, grid, iters, &ms[kCvtMulCvt]);
+  rc |= time_op<kRcpF64>(out, grid, iters, &ms[kRcpF64]);
+  if (rc) {
+    std::fprintf(stderr, "launch failed\n");
+    return 2;
+  }
+  std::printf("%s, %d CUs, max engine clock %d MHz: peak %.3e wave64 instructions/s (4 cycles each per SIMD)\n",
+              prop.gcnArchName, prop.multiProcessorCount, prop.clockRate / 1000, peak);
+  std::printf("%d workgroups of 256 threads (%d per CU), %d rounds of 8 independent chains per thread\n", grid,
+              wg_per_cu, iters);
+  for (int i = 0; i < kNumOps; ++i) {
+    const double rate = wave_instr / (ms[i] * 1e-3);
+    std::printf("  %-40s %8.4f ms  %.3e chain steps/s  %5.1f%% of peak  cost %.2f x v_fma_f32\n", kNames[i], ms[i],
+                rate, 100.0 * rate / peak, ms[i] / ms[kFmaF32]);
+  }
+  (void)hipFree(out);
+  return 0;
+}
