@@ -158,21 +158,29 @@ int launch_batches(p2z_server* sv) {
       nblk += r.nblocks;
       sv->pending.pop_front();
     }
-    P2Z_TRY(slot_reserve(s, nblk));
-    int64_t k = 0;
-    for (int64_t id : ids)
-      for (const auto& d : sv->reqs.at(id).blocks) s.h_desc[k++] = d;
-    std::stable_sort(s.h_desc, s.h_desc + nblk,
-                     [](const p2z_block_desc& a, const p2z_block_desc& b) { return a.nlayer > b.nlayer; });
-    const int tpw = o.tracks_per_wg ? o.tracks_per_wg : sv->ctx->cfg.tracks_per_wg;
-    const int wpc = o.waves_per_cu ? o.waves_per_cu : 8;
-    const int64_t grid = ((int64_t)sv->ctx->cus * wpc * 64 + tpw - 1) / tpw;
-    P2Z_HIP_CHECK(hipMemcpyAsync(s.d_desc, s.h_desc, nblk * sizeof(p2z_block_desc), hipMemcpyHostToDevice, s.stream));
-    P2Z_HIP_CHECK(p2z::launch_ragged(sv->trk, sv->hit, sv->arena_mem, sv->lanes, s.d_desc, nblk, o.width, o.math,
-                                     true, o.prefetch ? o.prefetch : 2, tpw, grid,
-                                     o.items_per_claim ? o.items_per_claim : 1, s.counter,
-                                     s.stream));
-    P2Z_HIP_CHECK(hipEventRecord(s.done, s.stream));
+    const int rc = [&]() -> int {
+      P2Z_TRY(slot_reserve(s, nblk));
+      int64_t k = 0;
+      for (int64_t id : ids)
+        for (const auto& d : sv->reqs.at(id).blocks) s.h_desc[k++] = d;
+      std::stable_sort(s.h_desc, s.h_desc + nblk,
+                       [](const p2z_block_desc& a, const p2z_block_desc& b) { return a.nlayer > b.nlayer; });
+      const int tpw = o.tracks_per_wg ? o.tracks_per_wg : sv->ctx->cfg.tracks_per_wg;
+      const int wpc = o.waves_per_cu ? o.waves_per_cu : 8;
+      const int64_t grid = ((int64_t)sv->ctx->cus * wpc * 64 + tpw - 1) / tpw;
+      P2Z_HIP_CHECK(
+          hipMemcpyAsync(s.d_desc, s.h_desc, nblk * sizeof(p2z_block_desc), hipMemcpyHostToDevice, s.stream));
+      P2Z_HIP_CHECK(p2z::launch_ragged(sv->trk, sv->hit, sv->arena_mem, sv->lanes, s.d_desc, nblk, o.width, o.math,
+                                       true, o.prefetch ? o.prefetch : 2, tpw, grid,
+                                       o.items_per_claim ? o.items_per_claim : 1, s.counter, s.stream));
+      P2Z_HIP_CHECK(hipEventRecord(s.done, s.stream));
+      return P2Z_OK;
+    }();
+    if (rc != P2Z_OK) {
+      // the batch did not start: its requests wait again, in their order
+      for (auto it = ids.rbegin(); it != ids.rend(); ++it) sv->pending.push_front(*it);
+      return rc;
+    }
     const double t = now_s();
     for (int64_t id : ids) sv->reqs.at(id).t_launch = t;
     s.requests = std::move(ids);
