@@ -616,13 +616,19 @@ __device__ inline void update_fp64gain(Track& t, const Hit& h, Trig* g = nullptr
   }
   const double res0 = (double)h.x - (double)p[0];
   const double res1 = (double)h.y - (double)p[1];
+  // p + K res as one FMA chain from p; the two angles keep their increment
+  // apart for the rotation of their sin/cos
   float dang[2] = {0.f, 0.f};
 #pragma unroll
   for (int a = 0; a < n; ++a) {
     const int i = L::at(a);
-    const double d = K0[a] * res0 + K1[a] * res1;
-    p[i] = (float)((double)p[i] + d);
-    if (i >= 4) dang[i - 4] = (float)d;
+    if (i >= 4) {
+      const double d = __builtin_fma(K0[a], res0, K1[a] * res1);
+      p[i] = (float)((double)p[i] + d);
+      dang[i - 4] = (float)d;
+    } else {
+      p[i] = (float)__builtin_fma(K0[a], res0, __builtin_fma(K1[a], res1, (double)p[i]));
+    }
   }
   if constexpr (kCarry) {
     float sd, omcd;
@@ -636,11 +642,12 @@ __device__ inline void update_fp64gain(Track& t, const Hit& h, Trig* g = nullptr
   // entry (a, b) = P_ab - K_a.(HP)_b + (K_a S - (P H^T)_a).K_b.
   // D_a = K_a S - (P H^T)_a vanishes for the exact optimal gain; computing it
   // keeps the form's insensitivity to errors in K.
+  // (two FMAs each, from -(P H^T)_a)
   double D0[6], D1[6];
 #pragma unroll
   for (int a = 0; a < n; ++a) {
-    D0[a] = K0[a] * r00 + K1[a] * r01 - q(a, 0);
-    D1[a] = K0[a] * r01 + K1[a] * r11 - q(a, 1);
+    D0[a] = __builtin_fma(K0[a], r00, __builtin_fma(K1[a], r01, -q(a, 0)));
+    D1[a] = __builtin_fma(K0[a], r01, __builtin_fma(K1[a], r11, -q(a, 1)));
   }
 #pragma unroll
   for (int a = 0; a < n; ++a)
