@@ -170,7 +170,9 @@ typedef struct {
                                     (descriptor order = processing order);
                                     otherwise one wave per 64 track slots     */
 #define P2Z_RAGGED_GRAPH 2u      /* launch through a captured hipGraph, cached
-                                    per argument set (either form)            */
+                                    per argument set (either form); a context
+                                    keeps up to 64 sets and empties its cache
+                                    (after its stream drains) when it is full */
 
 typedef struct {
   int32_t tracks_per_wg; /* 0: the context's setting                          */

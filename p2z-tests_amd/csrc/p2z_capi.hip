@@ -428,6 +428,12 @@ static int run_ragged_impl(p2z_ctx* c, const void* trk, const void* hit, void* o
                               persistent ? grid : -1, claim};
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
+    if (c->graphs.size() >= p2z_ctx::kMaxGraphs) {
+      // a full cache is emptied once the graphs launched from it have finished
+      P2Z_HIP_CHECK(hipStreamSynchronize(c->stream));
+      for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+      c->graphs.clear();
+    }
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     P2Z_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

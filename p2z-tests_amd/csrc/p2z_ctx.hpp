@@ -26,7 +26,8 @@ struct p2z_ctx {
   int cus = 0;
   using GraphKey = std::tuple<const void*, const void*, void*, const void*, const void*, int64_t, int, int, int,
                               int, int64_t, int>;
-  std::map<GraphKey, hipGraphExec_t> graphs;
+  std::map<GraphKey, hipGraphExec_t> graphs;  // at most kMaxGraphs argument sets
+  static constexpr size_t kMaxGraphs = 64;
 };
 
 namespace p2z {

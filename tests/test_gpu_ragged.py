@@ -133,6 +133,29 @@ def test_graph_replays_equal_eager(ctx, small_ragged, persistent, opts):
     d.free()
 
 
+def test_graph_cache_is_bounded(ctx, small_ragged):
+    """More distinct argument sets than a context caches graphs for (64): the
+    first k descriptors of the table, for 70 values of k, each captured and
+    launched.  The cache is emptied when full; every launch writes exactly the
+    output blocks of its k descriptors, equal to the eager launch's."""
+    w = small_ragged
+    W = w.width
+    d = Dev(ctx, w)
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, W, "fast")
+    eager = ctx.download(d.out).reshape(w.nblocks, -1)
+    out_of = np.asarray(w.blocks_lpt)["out_block"]
+    assert w.nblocks > 70
+    for k in range(1, 71):
+        ctx.h2d(d.out, np.zeros(w.tracks.size, np.float32))
+        ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, k, W, "fast", graph=True)
+        got = ctx.download(d.out).reshape(w.nblocks, -1)
+        done = np.zeros(w.nblocks, bool)
+        done[out_of[:k]] = True
+        assert np.array_equal(got[done].view(np.uint32), eager[done].view(np.uint32)), k
+        assert not got[~done].any(), k
+    d.free()
+
+
 @pytest.mark.parametrize("opts", [dict(prefetch=1), dict(tracks_per_wg=64), dict(waves_per_cu=2),
                                   dict(tracks_per_wg=128, prefetch=2, waves_per_cu=16),
                                   dict(items_per_claim=1), dict(items_per_claim=16, waves_per_cu=1),
