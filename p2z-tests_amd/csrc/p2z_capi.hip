@@ -341,35 +341,43 @@ static int run_host_impl(p2z_ctx* c, const void* trk, const void* hit, void* out
     c->slot_trk_bytes = need_trk;
     c->slot_hit_bytes = need_hit;
   }
-  for (int64_t k = 0; k < nchunks; ++k) {
-    const int64_t e0 = k * per, e1 = std::min(nevts, e0 + per);
-    if (e0 >= e1) break;
-    const int i = (int)(k % ns);
-    hipStream_t s = c->pipe[i];
-    const int64_t ne = e1 - e0;
-    P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_trk[i], (const char*)trk + e0 * trk_ev, ne * trk_ev,
-                                 hipMemcpyHostToDevice, s));
-    if (hit_pinned) {
-      // only the words the path reads: pos x, y, z and cov xx, xy, yy of each
-      // hit record (the record's last 3W words, cov xz, yz, zz, are never
-      // read, OMP:510-512); one row per record, same pitch on both sides
-      const size_t pitch = (size_t)p2z::kHitWords * width * sizeof(float);
-      P2Z_HIP_CHECK(hipMemcpy2DAsync(c->slot_hit[i], pitch, (const char*)hit + e0 * hit_ev, pitch,
-                                     (size_t)p2z::kHitUsedWords * width * sizeof(float),
-                                     (size_t)(ne * nb * nlayer), hipMemcpyHostToDevice, s));
-    } else {
-      // pageable memory: one contiguous copy (a strided copy from pageable
-      // memory goes row by row through the runtime's staging)
-      P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_hit[i], (const char*)hit + e0 * hit_ev, ne * hit_ev,
+  // every path out waits for the pipeline streams: no copy from or to the
+  // caller's host buffers is still in flight when this returns
+  const int rc = [&]() -> int {
+    for (int64_t k = 0; k < nchunks; ++k) {
+      const int64_t e0 = k * per, e1 = std::min(nevts, e0 + per);
+      if (e0 >= e1) break;
+      const int i = (int)(k % ns);
+      hipStream_t s = c->pipe[i];
+      const int64_t ne = e1 - e0;
+      P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_trk[i], (const char*)trk + e0 * trk_ev, ne * trk_ev,
                                    hipMemcpyHostToDevice, s));
+      if (hit_pinned) {
+        // only the words the path reads: pos x, y, z and cov xx, xy, yy of each
+        // hit record (the record's last 3W words, cov xz, yz, zz, are never
+        // read, OMP:510-512); one row per record, same pitch on both sides
+        const size_t pitch = (size_t)p2z::kHitWords * width * sizeof(float);
+        P2Z_HIP_CHECK(hipMemcpy2DAsync(c->slot_hit[i], pitch, (const char*)hit + e0 * hit_ev, pitch,
+                                       (size_t)p2z::kHitUsedWords * width * sizeof(float),
+                                       (size_t)(ne * nb * nlayer), hipMemcpyHostToDevice, s));
+      } else {
+        // pageable memory: one contiguous copy (a strided copy from pageable
+        // memory goes row by row through the runtime's staging)
+        P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_hit[i], (const char*)hit + e0 * hit_ev, ne * hit_ev,
+                                     hipMemcpyHostToDevice, s));
+      }
+      P2Z_HIP_CHECK(p2z::launch_fused((const float*)c->slot_trk[i], (const float*)c->slot_hit[i],
+                                      (float*)c->slot_out[i], ne * nb, nlayer, width, math, c->cfg, s));
+      P2Z_HIP_CHECK(hipMemcpyAsync((char*)out + e0 * trk_ev, c->slot_out[i], ne * trk_ev,
+                                   hipMemcpyDeviceToHost, s));
     }
-    P2Z_HIP_CHECK(p2z::launch_fused((const float*)c->slot_trk[i], (const float*)c->slot_hit[i],
-                                    (float*)c->slot_out[i], ne * nb, nlayer, width, math, c->cfg, s));
-    P2Z_HIP_CHECK(hipMemcpyAsync((char*)out + e0 * trk_ev, c->slot_out[i], ne * trk_ev,
-                                 hipMemcpyDeviceToHost, s));
+    return P2Z_OK;
+  }();
+  for (auto s : c->pipe) {
+    const hipError_t e = hipStreamSynchronize(s);
+    if (rc == P2Z_OK) P2Z_HIP_CHECK(e);
   }
-  for (auto s : c->pipe) P2Z_HIP_CHECK(hipStreamSynchronize(s));
-  return P2Z_OK;
+  return rc;
 }
 
 int p2z_run_host(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_t nevts, int64_t nb,

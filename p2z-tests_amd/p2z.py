@@ -395,6 +395,7 @@ class DeviceBuffer:
         if self.ptr:
             check(lib().p2z_dev_free(self.ctx.handle, self.ptr), "p2z_dev_free")
             self.ptr = 0
+            self.ctx._bufs = [b for b in self.ctx._bufs if b is not self]
 
 
 class Server:
@@ -549,10 +550,14 @@ class Context:
         return b
 
     def h2d(self, dst: DeviceBuffer, a: np.ndarray) -> None:
+        if a.nbytes > dst.nbytes:
+            raise ValueError(f"{a.nbytes} bytes do not fit a device buffer of {dst.nbytes}")
         check(lib().p2z_memcpy_h2d(self.handle, dst.ptr, _ptr(a), a.nbytes), "p2z_memcpy_h2d")
         self.sync()
 
     def d2h(self, a: np.ndarray, src: DeviceBuffer) -> np.ndarray:
+        if a.nbytes > src.nbytes:
+            raise ValueError(f"{a.nbytes} bytes requested from a device buffer of {src.nbytes}")
         check(lib().p2z_memcpy_d2h(self.handle, _ptr(a), src.ptr, a.nbytes), "p2z_memcpy_d2h")
         self.sync()
         return a
