@@ -96,6 +96,27 @@ def valu_roofline(math: str, tl_per_s: float):
             "frac": round(rate / VALU_PEAK, 4)}
 
 
+def host_cpu() -> dict:
+    """The host the CPU baseline runs on: CPU model, sockets, cores and
+    hardware threads (from /proc/cpuinfo), and the CPUs this process may use."""
+    info = {"model": None, "sockets": None, "cores": None, "threads": None,
+            "usable_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            text = f.read()
+    except OSError:
+        return info
+    procs = [p for p in text.split("\n\n") if p.strip()]
+    field = lambda p, k: next((ln.split(":", 1)[1].strip() for ln in p.splitlines() if ln.startswith(k)), None)
+    info["model"] = field(procs[0], "model name") if procs else None
+    phys = {(field(p, "physical id"), field(p, "core id")) for p in procs}
+    info["sockets"] = len({s for s, _ in phys})
+    info["cores"] = len(phys)
+    info["threads"] = len(procs)
+    info["smt"] = info["threads"] > info["cores"]
+    return info
+
+
 def cpu_baseline(nevts: int, ntrks: int, nlayer: int) -> dict | None:
     """The reference program (src/propagate-toz-test_OMP.cpp, built by
     oracle/Makefile from the reference's own source with its flags) on the
@@ -120,7 +141,8 @@ def cpu_baseline(nevts: int, ntrks: int, nlayer: int) -> dict | None:
             "cores": min(threads, ne), "kind": "reference",
             "sample": f"reference OpenMP program built from its source ({os.path.basename(exe)}), "
                       f"{ne} events x {nt} tracks x {nlayer} layers, NITER={niter}, "
-                      f"{threads} OpenMP threads over events, {secs:.3f} s timed"}
+                      f"{threads} OpenMP threads over events, {secs:.3f} s timed",
+            "host": host_cpu()}
 
 
 def cpu_baseline_port(nevts: int, ntrks: int, nlayer: int) -> dict | None:
@@ -141,7 +163,8 @@ def cpu_baseline_port(nevts: int, ntrks: int, nlayer: int) -> dict | None:
     return {"value": 2 * nevts * nb * W * nlayer / secs, "unit": "track-layer updates/s",
             "cores": min(threads, nevts), "kind": "port",
             "sample": f"oracle restatement, {nevts} events x {ntrks} tracks x {nlayer} layers, 2 passes, "
-                      f"{threads} threads over events, {secs:.3f} s"}
+                      f"{threads} threads over events, {secs:.3f} s",
+            "host": host_cpu()}
 
 
 def time_ms(ctx, fn, reps: int) -> float:
