@@ -119,3 +119,15 @@ def test_create_without_gpu_fails_cleanly():
     h = C.c_void_p()
     assert p2z.lib().p2z_create(C.byref(h), 0, None) == p2z.ENODEV
     assert "no GPU" in p2z.last_error()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with a message before
+    any GPU work (no context is created)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and not r.stdout.strip()
