@@ -34,7 +34,7 @@ import pytest
 
 import p2z
 import pyoracle as O
-from parity import compare, ok, random_blocks, track_major, within_envelope
+from parity import REF_WORDS_MIN, compare, ok, random_blocks, track_major, within_envelope
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -84,7 +84,7 @@ def test_ref_mode_is_the_reference_algorithm_bitwise(ctx, golden, name):
     eq = words_equal(out, exp)
     st = compare(track_major(out, W), track_major(exp, W))
     log(f"  words equal {eq:.6f}  {st}")
-    assert eq >= 0.9999 and ok(st), st
+    assert eq >= REF_WORDS_MIN and ok(st), st
 
 
 @pytest.mark.parametrize("W", [16, 32, 64, 128])
@@ -97,7 +97,7 @@ def test_ref_mode_bitwise_all_widths(ctx, W):
         exp = O.run(trk, hit, nevts, nb, W, nl)
     eq = words_equal(out, exp)
     log(f"  words equal {eq:.6f}")
-    assert eq >= 0.9999 and ok(compare(track_major(out, W), track_major(exp, W)))
+    assert eq >= REF_WORDS_MIN and ok(compare(track_major(out, W), track_major(exp, W)))
 
 
 def _steps(golden_steps, ctx, math):
@@ -132,7 +132,7 @@ def test_ref_mode_steps_bitwise(ctx, steps):
         eprop = np.concatenate([O.propagate_block(trk[b * 28 * W:(b + 1) * 28 * W], hit[b * 9 * W:(b + 1) * 9 * W], W)
                                 for b in range(nblk)])
     log(f"  propagate words equal {words_equal(prop, eprop):.6f}; update {words_equal(upd, steps['upd']):.6f}")
-    assert words_equal(prop, eprop) >= 0.9999
+    assert words_equal(prop, eprop) >= REF_WORDS_MIN
     assert words_equal(upd, steps["upd"]) == 1.0          # no libm in the update
 
 

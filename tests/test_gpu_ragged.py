@@ -8,7 +8,7 @@ import pytest
 
 import p2z
 import pyoracle as O
-from parity import compare
+from parity import REF_WORDS_MIN, compare
 
 pytestmark = pytest.mark.gpu
 
@@ -76,7 +76,7 @@ def test_ref_mode_bitwise_against_oracle(ctx, small_ragged, form):
     exp = oracle_ragged(w, variant="cr")
     eq = float(np.mean(out.view(np.uint32) == exp.view(np.uint32)))
     log(f"  words equal {eq:.6f}")
-    assert eq >= 0.9999
+    assert eq >= REF_WORDS_MIN
 
 
 def test_widths_give_the_same_tracks(ctx):
