@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import json
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -429,6 +430,32 @@ def test_dropin_program_per_block_loop():
     fused, per_block = report([]), report(["--per-block"])
     log("\n".join(per_block))
     assert len(fused) == 11 and per_block == fused
+
+
+def test_dropin_program_include_data_and_device_summary():
+    """bin/p2z_main's other modes print the default run's report.
+    --include-data streams host buffers through the pipelined copies every
+    iteration: same bits, so the same lines. --device-summary reduces the
+    report on the GPU in a different order: the same lines to within one unit
+    of the last printed digit (%f, 6 decimals)."""
+    args = [p2z.MAIN_PATH, "--nevts", "3", "--ntrks", "640", "--niter", "2", "--math", "ref"]
+
+    def report(extra):
+        res = subprocess.run(args + extra, capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stderr
+        return [ln for ln in res.stdout.splitlines()
+                if (ln.startswith("track ") and not ln.startswith("track in")) or ln.startswith("number of")]
+
+    base = report([])
+    assert len(base) == 11
+    assert report(["--include-data"]) == base
+    dev = report(["--device-summary"])
+    log("\n".join(dev))
+    num = re.compile(r"-?\d+\.\d+|-?\d+")
+    for a, b in zip(dev, base):
+        assert num.sub("#", a) == num.sub("#", b), (a, b)
+        for x, y in zip(num.findall(a), num.findall(b)):
+            assert abs(float(x) - float(y)) <= 1.5e-6, (a, b)
 
 
 def test_rare_branches_on_the_precision_workload(ctx):
