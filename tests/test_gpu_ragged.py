@@ -79,14 +79,15 @@ def test_ref_mode_bitwise_against_oracle(ctx, small_ragged, form):
     assert eq >= REF_WORDS_MIN
 
 
-def test_widths_give_the_same_tracks(ctx):
+@pytest.mark.parametrize("math", ["fast", "fp64gain"])
+def test_widths_give_the_same_tracks(ctx, math):
     """The packing is width independent (tracks sorted, then drawn), so every
     width must produce the same per-track results."""
     res = {}
     for W in (16, 32, 64, 128):
         w = p2z.Workload("ragged", nevents=6, width=W, seed=9)
         d = Dev(ctx, w)
-        ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, W, "fast")
+        ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, W, math)
         res[W] = by_track(w, ctx.download(d.out))
         d.free()
     for W in (16, 64, 128):
@@ -162,12 +163,13 @@ def test_graph_cache_is_bounded(ctx, small_ragged):
                                   SMALL_GRID, dict(SMALL_GRID, items_per_claim=3),
                                   dict(SMALL_GRID, items_per_claim=7, prefetch=1),
                                   dict(persistent=False, prefetch=1), dict(persistent=False, tracks_per_wg=256)])
-def test_launch_shapes_are_bitwise_equal(ctx, small_ragged, opts):
+@pytest.mark.parametrize("math", ["fast", "fp64gain"])
+def test_launch_shapes_are_bitwise_equal(ctx, small_ragged, opts, math):
     w = small_ragged
     d = Dev(ctx, w)
-    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast")
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, math)
     a = ctx.download(d.out)
-    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, "fast", **opts)
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, w.width, math, **opts)
     b = ctx.download(d.out)
     d.free()
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
