@@ -274,15 +274,18 @@ def test_driver_as_accurate_as_reference(ctx, golden, name, math):
     assert as_accurate(st, base), (st, base)
 
 
+@pytest.mark.parametrize("math", ["fast", "fp64gain"])
 @pytest.mark.parametrize("W", [16, 32, 64, 128])
-def test_fast_mode_all_widths_as_accurate(ctx, W):
+def test_fast_mode_all_widths_as_accurate(ctx, W, math):
+    """Both production modes at every width: error against the FP64 oracle no
+    larger than the reference FP32 algorithm's own."""
     nevts, ntrks, nl = 10, 960, 20
-    log(f"case accuracy/W={W}/fast")
+    log(f"case accuracy/W={W}/{math}")
     trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
     o64 = O.run_fp64(trk, hit, nevts, nb, W, nl)
     base = compare(track_major(O.run(trk, hit, nevts, nb, W, nl), W), o64)
-    st = compare(track_major(gpu_run(ctx, trk, hit, nevts, nb, nl, W, "fast"), W), o64)
-    log(f"  fast vs fp64: {st}\n  reference vs fp64: {base}")
+    st = compare(track_major(gpu_run(ctx, trk, hit, nevts, nb, nl, W, math), W), o64)
+    log(f"  {math} vs fp64: {st}\n  reference vs fp64: {base}")
     assert as_accurate(st, base), (st, base)
 
 
