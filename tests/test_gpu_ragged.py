@@ -79,6 +79,39 @@ def test_ref_mode_bitwise_against_oracle(ctx, small_ragged, form):
     assert eq >= REF_WORDS_MIN
 
 
+@pytest.mark.parametrize("persistent", [True, False])
+def test_descriptor_edge_cases_against_oracle(ctx, small_ragged, persistent):
+    """Descriptors cut to 0 layers (the track is copied through) and to 1
+    layer, beside full ones, so that waves mix lanes with no layers and lanes
+    with many: reference typing equals the oracle's ragged driver with the same
+    per-block layer counts.  An empty table launches nothing."""
+    w = small_ragged
+    blocks = np.array(w.blocks, copy=True)
+    blocks["nlayer"][::7] = 0
+    blocks["nlayer"][3::5] = np.minimum(blocks["nlayer"][3::5], 1)
+    assert (blocks["nlayer"] == 0).any() and (blocks["nlayer"] == 1).any()
+    d = Dev(ctx, w, lpt=False)
+    dblk = ctx.upload(np.ascontiguousarray(blocks))
+    ctx.h2d(d.out, np.zeros(w.tracks.size, np.float32))
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, dblk, w.nblocks, w.width, "ref", persistent=persistent)
+    out = ctx.download(d.out)
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, dblk, 0, w.width, "ref", persistent=persistent)   # no-op
+    again = ctx.download(d.out)
+    dblk.free()
+    d.free()
+    with O.use_variant("cr"):
+        exp = O.run_ragged(w.tracks, w.hits, w.nblocks, w.width, blocks["trk_block"], blocks["hit_block"],
+                           blocks["nlayer"], w.lane_nlayer)
+    eq = float(np.mean(out.view(np.uint32) == exp.view(np.uint32)))
+    log(f"case ragged-edges persistent={persistent}: words equal {eq:.6f}")
+    assert eq >= REF_WORDS_MIN
+    W = w.width
+    zero = np.flatnonzero(blocks["nlayer"] == 0)
+    trk_b, out_b = w.tracks.reshape(w.nblocks, -1), out.reshape(w.nblocks, -1)
+    assert np.array_equal(out_b[zero].view(np.uint32), trk_b[zero].view(np.uint32))
+    assert np.array_equal(again.view(np.uint32), out.view(np.uint32))
+
+
 @pytest.mark.parametrize("math", ["fast", "fp64gain"])
 def test_widths_give_the_same_tracks(ctx, math):
     """The packing is width independent (tracks sorted, then drawn), so every
