@@ -189,14 +189,13 @@ __device__ __forceinline__ void ragged_item(const float* __restrict__ trk, const
 // once.  Several items per claim keep the single counter's atomics
 // (serialised where they meet) few.  The counter starts at 0
 // (k_reset_counter runs first on the same stream).
+// FP64 gain: at most 128 VGPRs (4 waves per SIMD instead of 3); with the lane
+// index hidden below, what the limit spills is per-item state, not layer state.
 template <int W, int kMath, int kPrefetch>
-__global__ __launch_bounds__(256) void k_ragged_persistent(const float* __restrict__ trk,
-                                                           const float* __restrict__ hit,
-                                                           float* __restrict__ out,
-                                                           const int32_t* __restrict__ lane_nlay,
-                                                           const BlockDesc* __restrict__ blocks,
-                                                           int64_t nblocks, unsigned* __restrict__ counter,
-                                                           int claim) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMath == kFp64Gain ? 4 : 1))) void
+k_ragged_persistent(const float* __restrict__ trk, const float* __restrict__ hit, float* __restrict__ out,
+                    const int32_t* __restrict__ lane_nlay, const BlockDesc* __restrict__ blocks, int64_t nblocks,
+                    unsigned* __restrict__ counter, int claim) {
   const int64_t nitems = (nblocks * W + 63) / 64;
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / 64);
@@ -207,7 +206,12 @@ __global__ __launch_bounds__(256) void k_ragged_persistent(const float* __restri
     unsigned next = 0;
     for (int64_t item = first; item < last; ++item) {
       if (item == last - 1 && lane == 0) next = atomicAdd(counter, 1u);
-      ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, lane);
+      int lane_i = lane;
+      // FP64 gain: the lane index hidden from loop-invariant code motion, so
+      // that lane-dependent values are not hoisted out of the item loop into
+      // registers held across all of it (10 VGPRs)
+      if constexpr (kMath == kFp64Gain) asm volatile("" : "+v"(lane_i));
+      ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, lane_i);
     }
     chunk = nwaves + (int64_t)(unsigned)__shfl((int)next, 0);
   }

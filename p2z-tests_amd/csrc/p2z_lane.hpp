@@ -204,22 +204,34 @@ __device__ inline void update_ref(Track& t, const Hit& h) {
 // arguments like the track angles phi ~ 1.4 and theta ~ 2.8 are one ulp from
 // the nearest float); the tracks of an event share nearly the same angles, so
 // those ulps shift every track alike instead of averaging out.
+// A double constant placed in scalar registers where it is used.  FP64
+// instructions take no literal operands, so the compiler materialises such
+// constants in registers; left to itself it puts them in vector register pairs
+// and, in the persistent ragged kernel, keeps them live across its whole item
+// loop (16 VGPRs for the polynomials below, which run once per track).  The
+// empty asm leaves the value unchanged.
+__device__ __forceinline__ double sgpr_const(double v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+
 __device__ inline void sincos_accurate(float xf, float* sp, float* cp) {
   const double x = (double)xf;
-  const double k = __builtin_rint(x * 0.63661977236758134308);
-  const double r = __builtin_fma(-k, 6.123233995736766036e-17, __builtin_fma(-k, 1.5707963267948965580, x));
+  const double k = __builtin_rint(x * sgpr_const(0.63661977236758134308));
+  const double r = __builtin_fma(-k, sgpr_const(6.123233995736766036e-17),
+                                 __builtin_fma(-k, sgpr_const(1.5707963267948965580), x));
   const double u = r * r;
-  double ps = -2.480613491641723e-08;
-  ps = __builtin_fma(ps, u, 2.755599789925388e-06);
-  ps = __builtin_fma(ps, u, -0.0001984126694941195);
-  ps = __builtin_fma(ps, u, 0.008333333331134977);
-  ps = __builtin_fma(ps, u, -0.16666666666664143);
+  double ps = sgpr_const(-2.480613491641723e-08);
+  ps = __builtin_fma(ps, u, sgpr_const(2.755599789925388e-06));
+  ps = __builtin_fma(ps, u, sgpr_const(-0.0001984126694941195));
+  ps = __builtin_fma(ps, u, sgpr_const(0.008333333331134977));
+  ps = __builtin_fma(ps, u, sgpr_const(-0.16666666666664143));
   const double sr = __builtin_fma(r * u, ps, r);
-  double pc = -2.7237166798569765e-07;
-  pc = __builtin_fma(pc, u, 2.4799862612840642e-05);
-  pc = __builtin_fma(pc, u, -0.0013888885093552119);
-  pc = __builtin_fma(pc, u, 0.04166666663743458);
-  pc = __builtin_fma(pc, u, -0.4999999999996406);
+  double pc = sgpr_const(-2.7237166798569765e-07);
+  pc = __builtin_fma(pc, u, sgpr_const(2.4799862612840642e-05));
+  pc = __builtin_fma(pc, u, sgpr_const(-0.0013888885093552119));
+  pc = __builtin_fma(pc, u, sgpr_const(0.04166666663743458));
+  pc = __builtin_fma(pc, u, sgpr_const(-0.4999999999996406));
   const double cr = __builtin_fma(u, pc, 1.0);
   // quadrant q = k mod 4; rounding commutes with the swap and the signs
   const int q = (int)(long long)k & 3;
