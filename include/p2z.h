@@ -149,6 +149,13 @@ int p2z_run(p2z_ctx* ctx, const void* trk, const void* hit, void* out, int64_t n
 int p2z_run_host(p2z_ctx* ctx, const void* trk, const void* hit, void* out, int64_t nevts,
                  int64_t nb, int32_t nlayer, int32_t width, int32_t math);
 
+/* The sin and cos P2Z_MATH_REF computes (the bits of glibc's sinf / cosf /
+ * sincosf, which the reference calls at OMP:633-636, 643-644, 652-653) for
+ * the n consecutive float bit patterns first_bits, first_bits+1, ...:
+ * s[i], c[i] (device pointers, n floats each).  first_bits + n <= 2^32.
+ * Asynchronous.  A check of the device arithmetic against the C library. */
+int p2z_sincosf_ref(p2z_ctx* ctx, uint32_t first_bits, int64_t n, float* s, float* c);
+
 /* ---- ragged events ----------------------------------------------------------
  * No counterpart in the reference, whose sizes are uniform compile-time
  * macros (OMP:18-41).  Events of any track count, tracks of any layer count:

@@ -652,3 +652,21 @@ int oracle_format_summary(const oracle_summary_t* s, char* buf, int cap) {
                   fabs(s->stdz / s->avgz), s->avgdx, s->stddx, s->avgdy, s->stddy, s->avgdz,
                   s->stddz, s->avgpt, s->avgphi, s->avgtheta, s->nnans, s->nfail);
 }
+
+/* ---- the C library's sin / cos ----------------------------------------- */
+/* Through volatile pointers so that no compiler merges the pair into one
+ * sincosf call: this checks sinf and cosf themselves. */
+static float (*volatile libm_sinf)(float) = sinf;
+static float (*volatile libm_cosf)(float) = cosf;
+
+void oracle_libm_sincosf_range(uint32_t first_bits, int64_t n, float* s, float* c, int nthreads) {
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const uint32_t u = (uint32_t)(first_bits + (uint64_t)i);
+    float y;
+    memcpy(&y, &u, sizeof y);
+    s[i] = libm_sinf(y);
+    c[i] = libm_cosf(y);
+  }
+}

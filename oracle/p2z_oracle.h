@@ -99,6 +99,12 @@ void oracle_run_ragged(const float* trk, const float* hit, float* out, double* o
                        const int32_t* blk_nlay, const int32_t* lane_nlay, int fp64,
                        int nthreads);
 
+/* ---- the C library's sin / cos ------------------------------------------ */
+/* s[i] = sinf(x), c[i] = cosf(x) of the C library (separate calls, as the
+ * reference's source writes them at OMP:633-636, 643-644, 652-653) for the
+ * float bit patterns x = first_bits + i, i < n; OpenMP threads (<=0: all). */
+void oracle_libm_sincosf_range(uint32_t first_bits, int64_t n, float* s, float* c, int nthreads);
+
 /* ---- report (OMP:758-876) -------------------------------------------------- */
 typedef struct {
   double avgx, avgy, avgz, avgdx, avgdy, avgdz, avgpt, avgphi, avgtheta;

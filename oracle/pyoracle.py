@@ -59,6 +59,7 @@ _SIGS = {
     "oracle_run_ragged": ([vp, vp, vp, vp, i64, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int], None),
     "oracle_summary": ([vp, i64, i64, C.c_int, i64, C.POINTER(C.c_float), C.POINTER(OSummary)], None),
     "oracle_format_summary": ([C.POINTER(OSummary), C.c_char_p, C.c_int], C.c_int),
+    "oracle_libm_sincosf_range": ([C.c_uint32, i64, vp, vp, C.c_int], None),
 }
 
 _libs: dict = {}
@@ -227,6 +228,15 @@ def summary(out, nevts, nb, width, ntrks, lasthit) -> dict:
     last = (C.c_float * 3)(*[float(v) for v in lasthit])
     lib().oracle_summary(_p(out), nevts, nb, width, ntrks, last, C.byref(s))
     return s
+
+
+def libm_sincosf_range(first_bits: int, n: int, nthreads: int = 0):
+    """The C library's sinf and cosf for the float bit patterns first_bits ..
+    first_bits + n - 1; returns (s, c) float32 arrays."""
+    s = np.empty(n, np.float32)
+    c = np.empty(n, np.float32)
+    lib("glibc").oracle_libm_sincosf_range(first_bits, n, _p(s), _p(c), nthreads)
+    return s, c
 
 
 def format_summary(s: OSummary) -> str:

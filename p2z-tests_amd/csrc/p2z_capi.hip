@@ -294,6 +294,16 @@ int p2z_run(p2z_ctx* c, const void* trk, const void* hit, void* out, int64_t nev
   return P2Z_OK;
 }
 
+int p2z_sincosf_ref(p2z_ctx* c, uint32_t first_bits, int64_t n, float* s, float* co) {
+  if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
+  if (n < 0 || (uint64_t)first_bits + (uint64_t)n > (1ull << 32))
+    return p2z::fail(P2Z_EINVAL, "bit range %u + %lld leaves the 2^32 float patterns", first_bits, (long long)n);
+  if (n > 0 && (!s || !co)) return p2z::fail(P2Z_EINVAL, "null buffer");
+  P2Z_TRY(use_device(c));
+  P2Z_HIP_CHECK(p2z::launch_sincosf_range(first_bits, n, s, co, c->stream));
+  return P2Z_OK;
+}
+
 namespace {
 
 // True for page-locked host memory (hipHostMalloc); pageable memory makes

@@ -71,4 +71,7 @@ hipError_t launch_summary(const float* out, int64_t nevts, int64_t nb, int width
                           double hx, double hy, double hz, double* part, double* res,
                           hipStream_t s);
 
+// sin/cos of the reference-typing mode over consecutive float bit patterns.
+hipError_t launch_sincosf_range(uint32_t first, int64_t n, float* s, float* c, hipStream_t st);
+
 }  // namespace p2z

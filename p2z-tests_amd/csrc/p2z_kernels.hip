@@ -457,6 +457,19 @@ __global__ __launch_bounds__(kSumThreads) void k_summary_final(const double* __r
   }
 }
 
+// sin and cos of the reference-typing mode (sincosf_glibc) for the float bit
+// patterns first, first+1, ... (n of them, n <= 2^32 - first).
+__global__ __launch_bounds__(256) void k_sincosf_range(uint32_t first, int64_t n, float* __restrict__ s,
+                                                       float* __restrict__ c) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float y = __builtin_bit_cast(float, (uint32_t)(first + (uint64_t)i));
+  float sv, cv;
+  sincosf_glibc(y, &sv, &cv);
+  s[i] = sv;
+  c[i] = cv;
+}
+
 template <typename F>
 hipError_t dispatch_width(int width, F&& f) {
   switch (width) {
@@ -583,6 +596,12 @@ hipError_t launch_ragged(const float* trk, const float* hit, float* out, const i
       return hipGetLastError();
     });
   });
+}
+
+hipError_t launch_sincosf_range(uint32_t first, int64_t n, float* s, float* c, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sincosf_range, dim3((unsigned)grid_for(n, 256)), dim3(256), 0, st, first, n, s, c);
+  return hipGetLastError();
 }
 
 size_t summary_scratch_doubles(int64_t nevts, int64_t ntrks) {

@@ -9,12 +9,10 @@
 //               MultHelixPropTranspEndcap 343-373, KalmanUpdate_v2 503-620):
 //               FP32 with the FP64 sub-expressions the reference's `1.`
 //               literals and `(double)` casts create, no contraction, the
-//               reference's summation order, and correctly rounded sin/cos
-//               (evaluated in FP64, then rounded).  It reproduces the
-//               reference algorithm bit for bit under a correctly rounded
-//               libm; against glibc's sinf/cosf, which are not correctly
-//               rounded everywhere, it differs as the reference does when its
-//               own libm or FP model changes.
+//               reference's summation order, and sin/cos with the bits of
+//               the C library the reference calls (glibc's sinf/cosf,
+//               p2z_libm.hpp).  It reproduces the reference build bit for
+//               bit.
 //   kFast     : FP32 with FMA contraction, native reciprocals and the sparse
 //               structure of the Jacobian and of the covariance exploited.
 //   kFp64Gain : kFast propagation; the 2x2 residual-covariance inverse, the
@@ -33,6 +31,7 @@
 #include <stdint.h>
 
 #include "p2z_cov.hpp"
+#include "p2z_libm.hpp"
 
 namespace p2z {
 
@@ -58,16 +57,6 @@ struct Hit {
 // kRef: reference typing
 // ---------------------------------------------------------------------------
 
-// Correctly rounded single-precision sin and cos: the FP64 results are
-// within an FP64 ulp, so rounding them to FP32 gives the nearest float except
-// in cases of probability ~2^-28.
-__device__ inline void sincos_cr(float x, float* s, float* c) {
-  double sd, cd;
-  sincos((double)x, &sd, &cd);
-  *s = (float)sd;
-  *c = (float)cd;
-}
-
 __device__ inline void propagate_ref(Track& t, float hz) {
 #pragma clang fp contract(off)
   float* p = t.p;
@@ -78,22 +67,22 @@ __device__ inline void propagate_ref(Track& t, float hz) {
   const float ipt = p[3];
   const float pt = (float)(1. / (double)ipt);
   float sinP, cosP, sinT, cosT;
-  sincos_cr(p[4], &sinP, &cosP);
-  sincos_cr(p[5], &sinT, &cosT);
+  sincosf_glibc(p[4], &sinP, &cosP);
+  sincosf_glibc(p[5], &sinT, &cosT);
   const float pxin = cosP * pt;
   const float pyin = sinP * pt;
   const float icosT = (float)(1.0 / (double)cosT);
   const float icosTk = icosT / k;
   const float alpha = deltaZ * sinT * ipt * icosTk;
   float sina, cosa;
-  sincos_cr(alpha, &sina, &cosa);
+  sincosf_glibc(alpha, &sina, &cosa);
   p[0] = (float)((double)p[0] + (double)k * ((double)(pxin * sina) - (double)pyin * (1. - (double)cosa)));
   p[1] = (float)((double)p[1] + (double)k * ((double)(pyin * sina) + (double)pxin * (1. - (double)cosa)));
   p[2] = zout;
   p[4] = p[4] + alpha;
 
   float sC, cC;
-  sincos_cr(cosP * sina, &sC, &cC);
+  sincosf_glibc(cosP * sina, &sC, &cC);
   const double omcC = 1. - (double)cC;
   const float J02 = cosP * sinT * (sinP * cosa * sC - cosa) * icosT;
   const float J03 = (float)((double)(cosP * sinT * deltaZ * cosa) * (1. - (double)(sinP * sC)) * (double)(icosT * pt)
@@ -198,8 +187,12 @@ __device__ inline void update_ref(Track& t, const Hit& h) {
 //   least-squares polynomials of gen_sincos_coeffs.py (max relative error on
 //   |r| <= pi/4: sin 1.916e-14, cos 3.129e-13), then the quadrant k mod 4 selects and
 //   signs the two values.
-// The reduction error is |k| * 1.5e-33 (the next term of pi/2), below the
-// polynomial error for |x| < 2^60; NaN and infinity give NaN.
+// The reduction error is |k| * 1.5e-33 (the next term of pi/2), far below the
+// polynomial error while k is exact, i.e. while x * 2/pi rounds to the right
+// quadrant: true with a wide margin for |x| < 2^20 (k < 2^20; the product's
+// rounding error stays below 2^-32).  Larger |x| (never a track angle) take
+// the C-library-faithful sincosf_glibc, whose Payne-Hanek reduction is exact
+// at any magnitude; NaN and infinity give NaN.
 // The library's sincosf is faithful only (measured: 10-25% of its results at
 // arguments like the track angles phi ~ 1.4 and theta ~ 2.8 are one ulp from
 // the nearest float); the tracks of an event share nearly the same angles, so
@@ -216,6 +209,10 @@ __device__ __forceinline__ double sgpr_const(double v) {
 }
 
 __device__ inline void sincos_accurate(float xf, float* sp, float* cp) {
+  if (__builtin_fabsf(xf) >= 0x1p20f) {
+    sincosf_glibc(xf, sp, cp);
+    return;
+  }
   const double x = (double)xf;
   const double k = __builtin_rint(x * sgpr_const(0.63661977236758134308));
   const double r = __builtin_fma(-k, sgpr_const(6.123233995736766036e-17),

@@ -312,6 +312,7 @@ int p2z_server_pump(p2z_server* sv) {
 
 static int server_poll_impl(p2z_server* sv, p2z_completion* out, int32_t cap) {
   if (!sv || (!out && cap > 0)) return p2z::fail(P2Z_EINVAL, "null argument");
+  if (cap < 0) return p2z::fail(P2Z_EINVAL, "negative capacity %d", cap);
   const int n = std::min<int>(cap, (int)sv->ready.size());
   for (int i = 0; i < n; ++i) out[i] = sv->ready[i];
   sv->ready.erase(sv->ready.begin(), sv->ready.begin() + n);

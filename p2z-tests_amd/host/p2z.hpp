@@ -70,7 +70,7 @@ struct P2zConfig {
   int32_t nlayer = 20;
   int32_t niter = 10;
   double smear = 1e-7;
-  int32_t math = P2Z_MATH_FAST;
+  int32_t math = P2Z_MATH_REF;  // the reference's own arithmetic; --math fast to opt out
   int device = 0;
 };
 P2zConfig& p2zConfig();

@@ -6,7 +6,9 @@
 //            [--include-data] [--device-summary] [--per-block] [--device D]
 //
 // Defaults are the reference's (nevts 100, ntrks 9600, bsize 32, nlayer 20),
-// with NITER 10 as its Makefile sets it.  --include-data times host->device
+// with NITER 10 as its Makefile sets it, and its arithmetic (--math ref: the
+// same bits as the reference build, so the same printout; --math fast and
+// --math fp64gain are the production modes).  --include-data times host->device
 // and device->host copies of every pass (the reference's INCLUDE_DATA mode);
 // otherwise the timed region is the device-resident passes.  --per-block runs
 // the reference's own loop instead (OMP:733-747: per event and block, copy the

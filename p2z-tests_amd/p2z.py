@@ -171,6 +171,7 @@ _SIGS = {
     "p2z_summary_device": ([vp, vp, i64, i64, i32, i64, C.POINTER(C.c_float), C.POINTER(Summary)], i32),
     "p2z_summary_host": ([vp, i64, i64, i32, i64, C.POINTER(C.c_float), C.POINTER(Summary)], i32),
     "p2z_format_summary": ([C.POINTER(Summary), C.c_char_p, i32], i32),
+    "p2z_sincosf_ref": ([vp, u32, i64, vp, vp], i32),
 }
 
 
@@ -214,6 +215,14 @@ def _ptr(a) -> int:
     if isinstance(a, int):
         return a
     raise TypeError(f"cannot pass {type(a)} as a buffer")
+
+
+def _need(a, nbytes: int, what: str) -> None:
+    """Raises ValueError when a device buffer or array holds fewer than nbytes
+    (a raw int pointer is the caller's responsibility)."""
+    have = a.nbytes if isinstance(a, (DeviceBuffer, np.ndarray)) else None
+    if have is not None and have < nbytes:
+        raise ValueError(f"{what}: {nbytes} bytes needed, buffer holds {have}")
 
 
 # ---------------------------------------------------------------------------
@@ -576,12 +585,19 @@ class Context:
         return ms.value
 
     # the path
+    def _check_run(self, trk, hit, out, nevts, nb, nlayer, width) -> None:
+        _need(trk, 4 * trk_words(nevts * nb, width), "tracks")
+        _need(out, 4 * trk_words(nevts * nb, width), "out")
+        _need(hit, 4 * hit_words(nevts * nb, nlayer, width), "hits")
+
     def run(self, trk, hit, out, nevts: int, nb: int, nlayer: int, width: int, math="fast") -> None:
+        self._check_run(trk, hit, out, nevts, nb, nlayer, width)
         check(lib().p2z_run(self.handle, _ptr(trk), _ptr(hit), _ptr(out), nevts, nb, nlayer, width,
                             MATH.get(math, math)), "p2z_run")
 
     def run_host(self, trk: np.ndarray, hit: np.ndarray, out: np.ndarray, nevts: int, nb: int, nlayer: int,
                  width: int, math="fast") -> None:
+        self._check_run(trk, hit, out, nevts, nb, nlayer, width)
         check(lib().p2z_run_host(self.handle, _ptr(trk), _ptr(hit), _ptr(out), nevts, nb, nlayer, width,
                                  MATH.get(math, math)), "p2z_run_host")
 
@@ -591,16 +607,27 @@ class Context:
         track block (hit block b at b*hit_stride words)."""
         W = width
         hs = HIT_WORDS * W if hit_stride is None else hit_stride
+        self._check_blocks(trk, hit, nblocks, W, hs, out)
         f4 = 4
         check(lib().p2z_propagate_to_z(self.handle, trk.ptr + 6 * W * f4, trk.ptr, trk.ptr + 27 * W * f4, hit.ptr,
                                        out.ptr + 6 * W * f4, out.ptr, nblocks, TRK_WORDS * W, hs, W,
                                        MATH.get(math, math)), "p2z_propagate_to_z")
+
+    @staticmethod
+    def _check_blocks(trk, hit, nblocks: int, W: int, hs: int, out=None) -> None:
+        if nblocks <= 0:
+            return
+        _need(trk, 4 * trk_words(nblocks, W), "tracks")
+        if out is not None:
+            _need(out, 4 * trk_words(nblocks, W), "out")
+        _need(hit, 4 * ((nblocks - 1) * hs + HIT_WORDS * W), "hits")
 
     def kalman_update(self, trk: DeviceBuffer, hit: DeviceBuffer, nblocks: int, width: int, math="fast",
                       hit_stride: int | None = None) -> None:
         """KalmanUpdate_v2 in place on MPTRK blocks with one MPHIT block each."""
         W = width
         hs = HIT_WORDS * W if hit_stride is None else hit_stride
+        self._check_blocks(trk, hit, nblocks, W, hs)
         f4 = 4
         check(lib().p2z_kalman_update(self.handle, trk.ptr + 6 * W * f4, trk.ptr, hit.ptr + 3 * W * f4, hit.ptr,
                                       nblocks, TRK_WORDS * W, hs, W, MATH.get(math, math)), "p2z_kalman_update")
@@ -611,6 +638,7 @@ class Context:
         block each; fixed selects the textbook form (P2Z_V1_FIXED)."""
         W = width
         hs = HIT_WORDS * W if hit_stride is None else hit_stride
+        self._check_blocks(trk, hit, nblocks, W, hs)
         f4 = 4
         check(lib().p2z_kalman_update_v1(self.handle, trk.ptr + 6 * W * f4, trk.ptr, hit.ptr + 3 * W * f4, hit.ptr,
                                          nblocks, TRK_WORDS * W, hs, W, V1_FIXED if fixed else 0),
@@ -622,12 +650,21 @@ class Context:
         """All blocks of a descriptor table (device buffer of BlockDesc) through
         their layers: one persistent work-queue launch (optionally replayed from
         a captured hipGraph), or one wave per 64 track slots."""
+        _need(blocks, nblocks * C.sizeof(BlockDesc), "block descriptors")
         flags = (RAGGED_PERSISTENT if persistent else 0) | (RAGGED_GRAPH if graph else 0)
         o = RaggedOpts(tracks_per_wg, prefetch, waves_per_cu, items_per_claim)
         check(lib().p2z_run_ragged(self.handle, _ptr(trk), _ptr(hit), _ptr(out), _ptr(lane_nlayer), _ptr(blocks),
                                    nblocks, width, MATH.get(math, math), flags, C.byref(o)), "p2z_run_ragged")
 
+    def sincosf_ref(self, first_bits: int, n: int, s: DeviceBuffer, c: DeviceBuffer) -> None:
+        """sin and cos of the reference-typing mode for the float bit patterns
+        first_bits .. first_bits + n - 1, into device buffers s and c."""
+        _need(s, 4 * n, "sin out")
+        _need(c, 4 * n, "cos out")
+        check(lib().p2z_sincosf_ref(self.handle, first_bits, n, _ptr(s), _ptr(c)), "p2z_sincosf_ref")
+
     def summary_device(self, out: DeviceBuffer, nevts: int, nb: int, width: int, ntrks: int, lasthit) -> Summary:
+        _need(out, 4 * trk_words(nevts * nb, width), "out")
         s = Summary()
         last = (C.c_float * 3)(*[float(v) for v in lasthit])
         check(lib().p2z_summary_device(self.handle, out.ptr, nevts, nb, width, ntrks, last, C.byref(s)),

@@ -1,21 +1,18 @@
 """GPU parity: the HIP path through the C ABI against the pinned oracle and the
 reference build's golden vectors (metric: tests/parity.py).
 
-The reference's FP32 arithmetic is ill-conditioned over 20 layers: the same
-reference source changes per track by up to ~1.6e-4 in its parameters and
-~0.5 of the covariance floor when only its FP model (its own Makefile's FMA
-contraction) or its libm (correctly rounded sin/cos instead of glibc's)
-changes (tests/golden/ref_spread.json, measured by tests/golden/make_golden.py
-from the reference builds), and it is itself that far from exact arithmetic.
-So the modes are held to two different questions:
+The modes are held to two different questions:
 
   reference typing (the check mode: no contraction, the reference's FP64
-  sub-expressions) -- is it the reference?
-      - against the oracle under a correctly rounded libm: bitwise;
-      - single propagateToZ / KalmanUpdate_v2 steps against the reference
-        build (glibc): parameters 1e-5, covariance 1e-4;
-      - the 20-layer driver against the reference build: within 1.5x the
-        reference's own FMA / libm spread;
+  sub-expressions, sin/cos with glibc's bits) -- is it the reference?
+      - against the reference build's own outputs (tests/golden, built from
+        the reference source with contraction off, glibc sinf/cosf):
+        word for word, for single steps and for the 20-layer driver;
+      - against the oracle (the C restatement, pinned word for word to the
+        reference build): word for word at every width, at the full
+        100 x 9600 x 20 configuration, and on the config-0 inputs;
+      - per track within 1e-5 (parameters) / 1e-4 (covariance), which the
+        word-for-word equality implies; the pass fraction is logged;
       - the printed known answer, to the last printed digit;
   FP32 fast and FP64-gain (production modes) -- are they as accurate?
       - error against the full-FP64 oracle no larger than the reference FP32
@@ -25,7 +22,6 @@ So the modes are held to two different questions:
 """
 from __future__ import annotations
 
-import json
 import os
 import re
 import subprocess
@@ -35,7 +31,7 @@ import pytest
 
 import p2z
 import pyoracle as O
-from parity import REF_WORDS_MIN, compare, ok, random_blocks, track_major, within_envelope
+from parity import compare, ok, random_blocks, same_words, track_major, track_pass_fraction, words_equal
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -43,14 +39,6 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 def log(msg: str) -> None:
     print(msg, flush=True)
-
-
-def spread(name: str) -> dict:
-    """Envelope of a configuration: the larger of the FMA and libm spreads."""
-    with open(os.path.join(GOLDEN, "ref_spread.json")) as f:
-        d = json.load(f)
-    a, b = d[name], d[name + "_libm"]
-    return {k: max(a[k], b[k]) for k in ("par_max", "par_median", "cov_max", "cov_median")}
 
 
 def gpu_run(ctx, trk, hit, nevts, nb, nlayer, W, math):
@@ -64,28 +52,27 @@ def gpu_run(ctx, trk, hit, nevts, nb, nlayer, W, math):
     return out
 
 
-def words_equal(a: np.ndarray, b: np.ndarray) -> float:
-    return float(np.mean(a.view(np.uint32) == b.view(np.uint32)))
-
-
 # ---------------------------------------------------------------------------
 # strict tier
 # ---------------------------------------------------------------------------
 
 @pytest.mark.parametrize("name", ["small", "w16"])
-def test_ref_mode_is_the_reference_algorithm_bitwise(ctx, golden, name):
-    """Reference typing on the GPU equals the oracle (pinned bitwise to the
-    reference build) run with correctly rounded sin/cos, word for word."""
+def test_ref_mode_is_the_reference_build_bitwise(ctx, golden, name):
+    """Reference typing on the GPU equals the reference build's own 20-layer
+    outputs (tests/golden ref_small / ref_w16 'out': the reference source
+    compiled with contraction off, glibc sinf/cosf) and the oracle, word for
+    word; every track is within 1e-5 / 1e-4 of the reference."""
     g = golden[name]
     nevts, nb, W, nl = (int(g[k]) for k in ("nevts", "nb", "width", "nlayer"))
     log(f"case ref-bitwise/{name}")
     out = gpu_run(ctx, g["trk"], g["hit"], nevts, nb, nl, W, "ref")
-    with O.use_variant("cr"):
-        exp = O.run(g["trk"], g["hit"], nevts, nb, W, nl)
-    eq = words_equal(out, exp)
-    st = compare(track_major(out, W), track_major(exp, W))
-    log(f"  words equal {eq:.6f}  {st}")
-    assert eq >= REF_WORDS_MIN and ok(st), st
+    exp = O.run(g["trk"], g["hit"], nevts, nb, W, nl)
+    frac = track_pass_fraction(track_major(out, W), track_major(g["out"], W))
+    log(f"  words equal to the reference build {words_equal(out, g['out']):.6f}, to the oracle "
+        f"{words_equal(out, exp):.6f}; tracks within 1e-5/1e-4: {frac:.6f}")
+    assert same_words(out, g["out"])
+    assert same_words(out, exp)
+    assert frac == 1.0
 
 
 @pytest.mark.parametrize("W", [16, 32, 64, 128])
@@ -94,11 +81,9 @@ def test_ref_mode_bitwise_all_widths(ctx, W):
     log(f"case ref-bitwise/W={W}")
     trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
     out = gpu_run(ctx, trk, hit, nevts, nb, nl, W, "ref")
-    with O.use_variant("cr"):
-        exp = O.run(trk, hit, nevts, nb, W, nl)
-    eq = words_equal(out, exp)
-    log(f"  words equal {eq:.6f}")
-    assert eq >= REF_WORDS_MIN and ok(compare(track_major(out, W), track_major(exp, W)))
+    exp = O.run(trk, hit, nevts, nb, W, nl)
+    log(f"  words equal {words_equal(out, exp):.6f}")
+    assert same_words(out, exp)
 
 
 def _steps(golden_steps, ctx, math):
@@ -124,17 +109,16 @@ def steps():
 
 def test_ref_mode_steps_bitwise(ctx, steps):
     """Single propagateToZ / KalmanUpdate_v2 calls (random blocks, half with a
-    non-zero z row) in reference typing: bitwise equal to the oracle with a
-    correctly rounded libm, and the update bitwise equal to the reference."""
+    non-zero z row) in reference typing: bitwise equal to the reference
+    build's own steps and to the oracle."""
     W, prop, upd = _steps(steps, ctx, "ref")
     trk, hit = steps["trk"], steps["hit"]
     nblk = trk.size // (28 * W)
-    with O.use_variant("cr"):
-        eprop = np.concatenate([O.propagate_block(trk[b * 28 * W:(b + 1) * 28 * W], hit[b * 9 * W:(b + 1) * 9 * W], W)
-                                for b in range(nblk)])
-    log(f"  propagate words equal {words_equal(prop, eprop):.6f}; update {words_equal(upd, steps['upd']):.6f}")
-    assert words_equal(prop, eprop) >= REF_WORDS_MIN
-    assert words_equal(upd, steps["upd"]) == 1.0          # no libm in the update
+    eprop = np.concatenate([O.propagate_block(trk[b * 28 * W:(b + 1) * 28 * W], hit[b * 9 * W:(b + 1) * 9 * W], W)
+                            for b in range(nblk)])
+    log(f"  propagate words equal {words_equal(prop, steps['prop']):.6f}; update {words_equal(upd, steps['upd']):.6f}")
+    assert same_words(prop, steps["prop"]) and same_words(prop, eprop)
+    assert same_words(upd, steps["upd"])
 
 
 def test_ref_steps_strict_tolerance(ctx, steps):
@@ -215,9 +199,10 @@ def _parse(lines):
 def test_known_answer_full_size(ctx, golden, math):
     """The reference configuration end to end on the GPU (100 events x 9600
     tracks x 20 layers).
-    REF: the printed report equals the reference's to the last printed digit
-    (the reference's own builds and hosts differ by one unit there:
-    logs/propagate_gcc_omp.log vs the contraction-off build).
+    REF: the printed report is the contraction-off reference build's report,
+    character for character, and the published log's
+    (logs/propagate_gcc_omp.log:30-40) to the last printed digit (that build
+    host's FP model differs by one unit in y and pt).
     FAST, FP64GAIN: every printed value is at least as close to the FP64
     oracle's report as the reference's own value is (the reference's report
     carries an FP32 rounding bias, e.g. pt 2.0e-4 above the FP64 answer; the
@@ -230,9 +215,9 @@ def test_known_answer_full_size(ctx, golden, math):
     text = p2z.format_summary(s).splitlines()
     log("\n".join(text))
     if math == "ref":
-        for ref_lines in (golden["ref_summary_full"], golden["known_answer"]):
-            for a, b in zip(_parse(text), _parse(ref_lines)):
-                assert abs(a - b) <= 2.01e-6, (a, b)
+        assert text == golden["ref_summary_full"]
+        for a, b in zip(_parse(text), _parse(golden["known_answer"])):
+            assert abs(a - b) <= 1.01e-6, (a, b)
     else:
         for a, r, e in zip(_parse(text), _parse(golden["ref_summary_full"]), _parse(golden["fp64_summary_full"])):
             assert abs(a - e) <= max(abs(r - e), 2.01e-6), (a, r, e)
@@ -240,24 +225,53 @@ def test_known_answer_full_size(ctx, golden, math):
 
 
 # ---------------------------------------------------------------------------
-# envelope tier
+# full size and config 0 in reference typing
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("math", ["ref"])
-@pytest.mark.parametrize("name", ["small", "w16"])
-def test_driver_within_reference_spread(ctx, golden, name, math):
-    """20-layer driver in reference typing against the reference build's own
-    outputs (glibc, no contraction): within 1.5x the reference's FMA / libm
-    spread."""
-    g = golden[name]
-    nevts, nb, W, nl = (int(g[k]) for k in ("nevts", "nb", "width", "nlayer"))
-    log(f"case envelope/{name}/{math}")
-    out = gpu_run(ctx, g["trk"], g["hit"], nevts, nb, nl, W, math)
-    st = compare(track_major(out, W), track_major(g["out"], W))
-    env = spread(name)
-    log(f"  {st}\n  envelope {env}")
-    assert within_envelope(st, env), (st, env)
+def test_ref_mode_full_size_per_track(ctx):
+    """The full configuration (100 events x 9600 tracks x 20 layers, W=32, the
+    reference's own inputs) in reference typing: every track equals the
+    oracle (the reference algorithm with glibc, run on all host cores) word
+    for word."""
+    nevts, ntrks, W, nl = 100, 9600, 32, 20
+    log(f"case ref-full-size {nevts}x{ntrks}x{nl}")
+    trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
+    out = gpu_run(ctx, trk, hit, nevts, nb, nl, W, "ref")
+    exp = O.run(trk, hit, nevts, nb, W, nl)
+    frac = track_pass_fraction(track_major(out, W), track_major(exp, W))
+    log(f"  words equal {words_equal(out, exp):.8f}; tracks within 1e-5/1e-4: {frac:.8f}")
+    assert same_words(out, exp)
 
+
+def test_config0_all_modes(ctx):
+    """Config 0: the nominal track and hit of propagate-toz-pseudocode.txt:2-15
+    smeared by 1e-5, 10 events x 9600 tracks x 20 layers.  Reference typing
+    equals the oracle word for word.  The pseudocode's covariance is not
+    positive definite (its (5,3) entry is 1.1 times sqrt(P33 P55)), and the
+    same hit on every layer repeats the update 19 times at deltaZ = 0, so the
+    filter is chaotic on it: the reference FP32 algorithm itself departs from
+    FP64 by more than 1% on about 57% of the tracks (up to 0.47 in ipt).  The
+    production modes are held to the reference's per-track 99th percentiles
+    (+25%) and medians (+10%) against the FP64 oracle, not to single-track
+    maxima."""
+    nevts, ntrks, W, nl = 10, 9600, 32, 20
+    log(f"case config0 {nevts}x{ntrks}x{nl}")
+    trk, hit, last, nb = p2z.generate("config0", nevts, ntrks, W, nl)
+    exp = O.run(trk, hit, nevts, nb, W, nl)
+    out = gpu_run(ctx, trk, hit, nevts, nb, nl, W, "ref")
+    log(f"  ref words equal {words_equal(out, exp):.8f}")
+    assert same_words(out, exp)
+    o64 = O.run_fp64(trk, hit, nevts, nb, W, nl)
+    base = compare(track_major(exp, W), o64)
+    for math in ("fast", "fp64gain"):
+        st = compare(track_major(gpu_run(ctx, trk, hit, nevts, nb, nl, W, math), W), o64)
+        log(f"  {math} vs fp64: {st}\n  reference vs fp64: {base}")
+        assert as_accurate(st, base, slack_max=float("inf")), (math, st, base)
+
+
+# ---------------------------------------------------------------------------
+# accuracy tier
+# ---------------------------------------------------------------------------
 
 @pytest.mark.parametrize("math", ["fast", "fp64gain"])
 @pytest.mark.parametrize("name", ["small", "w16"])
@@ -325,6 +339,52 @@ def test_device_summary_matches_host(ctx):
     assert p2z.format_summary(sd) == p2z.format_summary(sh)
 
 
+@pytest.mark.parametrize("nevts,ntrks", [(10, 9600), (100, 9600)])
+def test_device_summary_equals_oracle(ctx, golden, nevts, ntrks):
+    """The report reduced on the GPU (p2z_summary_device, OMP:758-876) from the
+    reference-typing output against the oracle's report of the oracle's own
+    output (the same words): every statistic within 1e-12 relative (the
+    device sums in another order) and the same printed lines; at the full
+    configuration, the reference build's printed report."""
+    W, nl = 32, 20
+    log(f"case device-summary {nevts}x{ntrks}")
+    trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
+    dt, dh, do = ctx.upload(trk), ctx.upload(hit), ctx.alloc(trk.nbytes)
+    ctx.run(dt, dh, do, nevts, nb, nl, W, "ref")
+    sd = ctx.summary_device(do, nevts, nb, W, ntrks, last)
+    for b in (dt, dh, do):
+        b.free()
+    so = O.summary(O.run(trk, hit, nevts, nb, W, nl), nevts, nb, W, ntrks, last).as_dict()
+    a = sd.as_dict()
+    for k in so:
+        assert abs(a[k] - so[k]) <= 1e-12 * max(1.0, abs(so[k])), (k, a[k], so[k])
+    text = p2z.format_summary(sd)
+    assert text == O.format_summary(O.summary(O.run(trk, hit, nevts, nb, W, nl), nevts, nb, W, ntrks, last))
+    if nevts == 100:
+        assert text.splitlines() == golden["ref_summary_full"]
+
+
+@pytest.mark.parametrize("pinned", [False, True])
+def test_host_pipeline_equals_oracle(ctx, pinned):
+    """The include-data pipeline (CU4:830-875: chunks of events copied in,
+    computed and copied out over several streams) in reference typing equals
+    the oracle word for word, from pageable and from pinned host buffers,
+    with more chunks than streams and a short last chunk."""
+    nevts, ntrks, W, nl = 23, 1024, 32, 20
+    trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
+    exp = O.run(trk, hit, nevts, nb, W, nl)
+    log(f"case run_host/oracle pinned={pinned}")
+    if pinned:
+        ptrk, phit, pout = ctx.pinned_copy(trk), ctx.pinned_copy(hit), ctx.pinned_empty(trk.size)
+        ctx.run_host(ptrk, phit, pout, nevts, nb, nl, W, "ref")
+        out = np.array(pout)
+    else:
+        out = np.empty_like(trk)
+        ctx.run_host(trk, hit, out, nevts, nb, nl, W, "ref")
+    log(f"  words equal {words_equal(out, exp):.6f}")
+    assert same_words(out, exp)
+
+
 def test_host_pipeline_equals_device_run(ctx):
     """The include-data pipeline (chunks over streams) computes the same bits."""
     nevts, ntrks, W, nl = 13, 1024, 32, 20
@@ -378,9 +438,8 @@ def test_edge_sizes(ctx):
     for nl in (1, 2):
         trk, hit, last, nb = p2z.generate("ref", 1, 64, W, nl)
         out = gpu_run(ctx, trk, hit, 1, nb, nl, W, "ref")
-        with O.use_variant("cr"):
-            exp = O.run(trk, hit, 1, nb, W, nl)
-        assert words_equal(out, exp) >= 0.999, nl
+        exp = O.run(trk, hit, 1, nb, W, nl)
+        assert same_words(out, exp), nl
     trk, hit, last, nb = p2z.generate("ref", 1, 64, W, 1)
     out = gpu_run(ctx, trk, hit, 1, nb, 0, W, "fast")
     assert np.array_equal(out.view(np.uint32), trk.view(np.uint32))
@@ -400,8 +459,8 @@ def test_invalid_arguments_do_not_launch(ctx):
 
 def test_dropin_program_report(golden):
     """bin/p2z_main, the reference program's drop-in, in reference typing:
-    its report equals the oracle's (correctly rounded libm) on the same
-    inputs, and its formatted line has the reference's fields."""
+    its report equals the oracle's on the same inputs, and its formatted
+    line has the reference's fields."""
     nevts, ntrks = 4, 960
     res = subprocess.run([p2z.MAIN_PATH, "--nevts", str(nevts), "--ntrks", str(ntrks), "--niter", "2",
                           "--math", "ref"], capture_output=True, text=True, timeout=120)
@@ -411,10 +470,21 @@ def test_dropin_program_report(golden):
     assert any(ln.startswith("formatted 2 4 960 32 30 ") for ln in lines)
     rep = [ln for ln in lines if (ln.startswith("track ") and not ln.startswith("track in")) or ln.startswith("number of")]
     trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, 32, 20)
-    with O.use_variant("cr"):
-        s = O.summary(O.run(trk, hit, nevts, nb, 32, 20), nevts, nb, 32, ntrks, last)
-        expect = O.format_summary(s).splitlines()
+    s = O.summary(O.run(trk, hit, nevts, nb, 32, 20), nevts, nb, 32, ntrks, last)
+    expect = O.format_summary(s).splitlines()
     assert rep == expect
+
+
+def test_dropin_program_default_prints_reference_report(golden):
+    """bin/p2z_main with no options runs the reference's configuration
+    (100 events x 9600 tracks x 20 layers, its inputs) in its arithmetic and
+    prints the reference build's report, line for line."""
+    res = subprocess.run([p2z.MAIN_PATH, "--niter", "1"], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    rep = [ln for ln in res.stdout.splitlines()
+           if (ln.startswith("track ") and not ln.startswith("track in")) or ln.startswith("number of")]
+    log("\n".join(rep))
+    assert rep == golden["ref_summary_full"]
 
 
 def test_dropin_program_per_block_loop():
@@ -487,9 +557,8 @@ def test_rare_branches_on_the_precision_workload(ctx):
     nev, nbe = 2, nb // 2
     o64 = O.run_fp64(trk, hit, nev, nbe, W, nl)
     base = compare(track_major(O.run(trk, hit, nev, nbe, W, nl), W), o64)
-    with O.use_variant("cr"):
-        ref_oracle = O.run(trk, hit, nev, nbe, W, nl)
-    assert words_equal(gpu_run(ctx, trk, hit, nev, nbe, nl, W, "ref"), ref_oracle) == 1.0
+    ref_oracle = O.run(trk, hit, nev, nbe, W, nl)
+    assert same_words(gpu_run(ctx, trk, hit, nev, nbe, nl, W, "ref"), ref_oracle)
     # beyond the median this workload is chaotic for the FP32 algorithm itself
     # (the reference's p99 parameter error against FP64 is O(10)): medians only
     res = {}

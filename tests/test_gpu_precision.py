@@ -40,7 +40,8 @@ def quantiles(x: np.ndarray) -> dict:
 
 
 def test_precision_report(ctx):
-    nevents, ntrk = 4, 9600
+    """The full precision configuration: 100 events x 9600 tracks x 20 layers."""
+    nevents, ntrk = 100, 9600
     w = p2z.Workload("precision", nevents=nevents, width=32, seed=42, ntracks=ntrk, nlayer=20)
     b = w.blocks
     o64 = O.run_ragged(w.tracks, w.hits, w.nblocks, 32, b["trk_block"], b["hit_block"], b["nlayer"],

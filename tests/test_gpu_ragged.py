@@ -8,7 +8,7 @@ import pytest
 
 import p2z
 import pyoracle as O
-from parity import REF_WORDS_MIN, compare
+from parity import compare, same_words, words_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -30,11 +30,10 @@ def by_track(w: "p2z.Workload", out: np.ndarray) -> np.ndarray:
     return r
 
 
-def oracle_ragged(w, fp64=False, variant="glibc"):
+def oracle_ragged(w, fp64=False):
     b = w.blocks
-    with O.use_variant(variant):
-        res = O.run_ragged(w.tracks, w.hits, w.nblocks, w.width, b["trk_block"], b["hit_block"], b["nlayer"],
-                           w.lane_nlayer, fp64=fp64)
+    res = O.run_ragged(w.tracks, w.hits, w.nblocks, w.width, b["trk_block"], b["hit_block"], b["nlayer"],
+                       w.lane_nlayer, fp64=fp64)
     if fp64:
         keep = np.zeros(w.nblocks * w.width, bool)
         for e in range(w.nevents):
@@ -73,10 +72,9 @@ def test_ref_mode_bitwise_against_oracle(ctx, small_ragged, form):
                    persistent=form != "oneshot", graph=form == "graph")
     out = ctx.download(d.out)
     d.free()
-    exp = oracle_ragged(w, variant="cr")
-    eq = float(np.mean(out.view(np.uint32) == exp.view(np.uint32)))
-    log(f"  words equal {eq:.6f}")
-    assert eq >= REF_WORDS_MIN
+    exp = oracle_ragged(w)
+    log(f"  words equal {words_equal(out, exp):.6f}")
+    assert same_words(out, exp)
 
 
 @pytest.mark.parametrize("persistent", [True, False])
@@ -99,12 +97,10 @@ def test_descriptor_edge_cases_against_oracle(ctx, small_ragged, persistent):
     again = ctx.download(d.out)
     dblk.free()
     d.free()
-    with O.use_variant("cr"):
-        exp = O.run_ragged(w.tracks, w.hits, w.nblocks, w.width, blocks["trk_block"], blocks["hit_block"],
-                           blocks["nlayer"], w.lane_nlayer)
-    eq = float(np.mean(out.view(np.uint32) == exp.view(np.uint32)))
-    log(f"case ragged-edges persistent={persistent}: words equal {eq:.6f}")
-    assert eq >= REF_WORDS_MIN
+    exp = O.run_ragged(w.tracks, w.hits, w.nblocks, w.width, blocks["trk_block"], blocks["hit_block"],
+                       blocks["nlayer"], w.lane_nlayer)
+    log(f"case ragged-edges persistent={persistent}: words equal {words_equal(out, exp):.6f}")
+    assert same_words(out, exp)
     W = w.width
     zero = np.flatnonzero(blocks["nlayer"] == 0)
     trk_b, out_b = w.tracks.reshape(w.nblocks, -1), out.reshape(w.nblocks, -1)
