@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define P2Z_ABI_VERSION 1
+#define P2Z_ABI_VERSION 2
 
 /* status codes */
 #define P2Z_OK 0
@@ -275,7 +275,21 @@ int p2z_workload_load(const char* path, p2z_workload** out);
  * overlaps on the device; it also collects finished batches.  Results stay
  * in the arena until p2z_server_release.  Per-lane arithmetic does not depend
  * on the batch: a request's result is the same alone and in any mix.
+ * With P2Z_SERVE_RESIDENT the requests are served by a device-resident loop
+ * instead: one persistent kernel, launched when the server is created, stays
+ * on the GPU and takes work as it is published.  p2z_server_submit writes an
+ * admitted request's work items (64 track slots each, descriptors
+ * longest-first) into a ring in pinned host memory and publishes them with
+ * one store; the kernel's dispatcher wave sees them over PCIe and its worker
+ * waves compute them as they come, so no launch or copy stands between a
+ * request and the GPU; the worker that finishes a request's last item writes
+ * a completion record into host memory, which p2z_server_pump collects.  The
+ * loop ends when the server is destroyed, or after about two seconds in which
+ * neither submit nor pump is called (the next call launches it again).  At
+ * most 4096 requests and 65536 work items are in flight at once (P2Z_EBUSY
+ * beyond); nstreams and max_batch_blocks do not apply.
  * A server is used by one host thread at a time. */
+#define P2Z_SERVE_RESIDENT 1u
 typedef struct {
   int64_t budget_bytes;     /* output arena size                               */
   int32_t nstreams;         /* batches in flight at once, 1..16; 0 = 4         */
@@ -287,6 +301,8 @@ typedef struct {
   int64_t pool_blocks;      /* MPTRK blocks in trk (W-lane rows of lane_nlayer) */
   int64_t pool_hit_blocks;  /* MPHIT blocks in hit; requests are checked
                                against both (p2z_check_blocks) on submit     */
+  uint32_t flags;           /* P2Z_SERVE_RESIDENT or 0 (batches of launches) */
+  int32_t reserved;         /* 0                                             */
 } p2z_server_opts;
 
 typedef struct {
@@ -315,6 +331,7 @@ typedef struct {
   double tl_per_s;                           /* track-layers / t_span           */
   double lat_p50, lat_p99, lat_p999, lat_mean, lat_max; /* seconds            */
   double mean_batch_events;
+  double host_submit_s, host_pump_s;         /* host time inside submit / pump  */
 } p2z_serve_report;
 
 /* Poisson arrivals at `rate` events/s of the pool's events (event e = blocks

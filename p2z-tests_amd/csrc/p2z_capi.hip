@@ -27,6 +27,7 @@ int use_device(p2z_ctx* c) {
 }
 }  // namespace p2z
 
+
 namespace {
 using p2z::use_device;
 
@@ -476,25 +477,25 @@ int p2z_run_ragged(p2z_ctx* c, const void* trk, const void* hit, void* out, cons
   });
 }
 
-int p2z_summary_device(p2z_ctx* c, const void* out, int64_t nevts, int64_t nb, int32_t width,
-                       int64_t ntrks, const float lasthit[3], p2z_summary* s) {
+namespace {
+
+int summary_args(p2z_ctx* c, const void* out, int64_t nevts, int64_t nb, int32_t width, int64_t ntrks,
+                 const float* lasthit, p2z_summary* s) {
   if (!c || !out || !lasthit || !s) return p2z::fail(P2Z_EINVAL, "null argument");
   if (!p2z::valid_width(width)) return p2z::fail(P2Z_EINVAL, "width must be 16/32/64/128 (got %d)", width);
   if (nevts <= 0 || ntrks <= 0 || ntrks > nb * width)
     return p2z::fail(P2Z_EINVAL, "need nevts > 0 and 0 < ntrks <= nb*width");
-  P2Z_TRY(use_device(c));
-  const size_t need = p2z::summary_scratch_doubles(nevts, ntrks);
-  if (c->sum_part_len < need) {
-    if (c->sum_part) P2Z_HIP_CHECK(hipFree(c->sum_part));
-    c->sum_part = nullptr;
-    c->sum_part_len = 0;
-    P2Z_HIP_CHECK(hipMalloc((void**)&c->sum_part, need * sizeof(double)));
-    c->sum_part_len = need;
-  }
+  return P2Z_OK;
+}
+
+// Runs the report kernels with the given scratch, then copies the 18 results
+// to the host and unpacks them.
+int summary_run(p2z_ctx* c, const void* out, int64_t nevts, int64_t nb, int32_t width, int64_t ntrks,
+                const float* lasthit, double* part, double* res, p2z_summary* s) {
   P2Z_HIP_CHECK(p2z::launch_summary((const float*)out, nevts, nb, width, ntrks, lasthit[0], lasthit[1],
-                                    lasthit[2], c->sum_part, c->sum_res, c->stream));
+                                    lasthit[2], part, res, c->stream));
   double r[18];
-  P2Z_HIP_CHECK(hipMemcpyAsync(r, c->sum_res, sizeof r, hipMemcpyDeviceToHost, c->stream));
+  P2Z_HIP_CHECK(hipMemcpyAsync(r, res, sizeof r, hipMemcpyDeviceToHost, c->stream));
   P2Z_HIP_CHECK(hipStreamSynchronize(c->stream));
   std::memset(s, 0, sizeof *s);
   s->avgx = r[0]; s->avgy = r[1]; s->avgz = r[2];
@@ -504,6 +505,23 @@ int p2z_summary_device(p2z_ctx* c, const void* out, int64_t nevts, int64_t nb, i
   s->stdx = r[12]; s->stdy = r[13]; s->stdz = r[14];
   s->stddx = r[15]; s->stddy = r[16]; s->stddz = r[17];
   return P2Z_OK;
+}
+
+}  // namespace
+
+int p2z_summary_device(p2z_ctx* c, const void* out, int64_t nevts, int64_t nb, int32_t width,
+                       int64_t ntrks, const float lasthit[3], p2z_summary* s) {
+  P2Z_TRY(summary_args(c, out, nevts, nb, width, ntrks, lasthit, s));
+  P2Z_TRY(use_device(c));
+  const size_t need = p2z::summary_scratch_doubles(nevts, ntrks);
+  if (c->sum_part_len < need) {
+    if (c->sum_part) P2Z_HIP_CHECK(hipFree(c->sum_part));
+    c->sum_part = nullptr;
+    c->sum_part_len = 0;
+    P2Z_HIP_CHECK(hipMalloc((void**)&c->sum_part, need * sizeof(double)));
+    c->sum_part_len = need;
+  }
+  return summary_run(c, out, nevts, nb, width, ntrks, lasthit, c->sum_part, c->sum_res, s);
 }
 
 }  // extern "C"

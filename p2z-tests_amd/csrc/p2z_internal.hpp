@@ -71,6 +71,75 @@ hipError_t launch_summary(const float* out, int64_t nevts, int64_t nb, int width
                           double hx, double hy, double hz, double* part, double* res,
                           hipStream_t s);
 
+// ---- device-resident serving loop -------------------------------------------
+// Work item of the resident loop: 64 track slots of one request.  W <= 64:
+// up to 64/W descriptors (slot s of the item is lane s % W of desc[s / W]);
+// W = 128: desc[0] only, lanes half*64 .. half*64+63.  Descriptors past
+// ndesc are padding (their slots compute nothing).
+struct ServeItem {
+  BlockDesc desc[4];
+  int32_t ndesc;
+  int32_t half;
+  int32_t req_slot;   // completion slot of the request
+  int32_t req_items;  // items of the request
+};
+static_assert(sizeof(ServeItem) == 144, "serve item layout");
+
+// Completion record, written by the device into host memory: seq (= its
+// index + 1) is written last, after the record, so the host reads a record
+// only once seq says it is complete.
+struct ServeCompletion {
+  int32_t req_slot;
+  int32_t pad;
+  uint64_t seq;
+};
+
+// Control words in host memory (coherent pinned), one per 64-byte line.
+struct ServeCtl {
+  uint64_t tail;  // items published by the host (item k lives at k % cap)
+  uint64_t pad0[7];
+  uint64_t heartbeat;  // advanced by the host while it serves
+  uint64_t pad1[7];
+  uint32_t stop;    // 1: end the loop once the published items are done
+  uint32_t epoch;   // launch number, set by the host before each launch
+  uint32_t exited;  // the dispatcher copies epoch here when it leaves
+  uint32_t pad2[13];
+};
+
+// Device-side state of the loop (device memory, zeroed before the first launch).
+struct ServeDev {
+  uint64_t head;       // items claimed by workers
+  uint64_t pad0[7];
+  uint64_t dev_tail;   // copy of ServeCtl::tail kept by the dispatcher wave
+  uint64_t pad1[7];
+  uint32_t dev_stop;   // set by the dispatcher: workers end once nothing is left
+  uint32_t pad2[15];
+  uint64_t comp_tail;  // completion records written
+  uint64_t pad3[7];
+  // followed by the per-request item counters (uint32_t done[req_slots])
+};
+
+struct ServeLaunch {
+  const float* trk;
+  const float* hit;
+  float* out;
+  const int32_t* lanes;
+  const ServeItem* items;    // host ring (device-visible pointer)
+  int64_t item_cap;
+  ServeCtl* ctl;             // host control words (device-visible pointer)
+  ServeCompletion* comps;    // host completion ring (device-visible pointer)
+  int64_t comp_cap;
+  ServeDev* dev;             // device state
+  uint32_t* done;            // per request slot: items finished (device)
+  uint64_t idle_ticks;       // heartbeat timeout, in s_memrealtime ticks (100 MHz)
+};
+
+// One persistent launch of the resident loop (grid workgroups of tpw
+// threads; wave 0 of workgroup 0 is the dispatcher).  Returns when launched;
+// the kernel runs until ServeCtl::stop or the heartbeat timeout.
+hipError_t launch_serve_resident(const ServeLaunch& a, int width, int math, int prefetch, int tpw, int64_t grid,
+                                 hipStream_t s);
+
 // sin/cos of the reference-typing mode over consecutive float bit patterns.
 hipError_t launch_sincosf_range(uint32_t first, int64_t n, float* s, float* c, hipStream_t st);
 

@@ -232,6 +232,194 @@ __global__ __launch_bounds__(256) void k_ragged_oneshot(const float* __restrict_
   ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, threadIdx.x & 63);
 }
 
+// ---------------------------------------------------------------------------
+// device-resident serving loop
+// ---------------------------------------------------------------------------
+// One persistent launch serves requests as the host publishes them: the host
+// writes work items (ServeItem, 64 track slots of one request) into a ring in
+// coherent pinned host memory and advances ServeCtl::tail; wave 0 of
+// workgroup 0 (the dispatcher) polls that word over PCIe and copies it into
+// device memory (ServeDev::dev_tail), where the worker waves wait for it.  A
+// worker claims item indexes in order (ServeDev::head), computes the item as
+// the ragged pass does, and counts it against its request; the worker that
+// completes a request writes a completion record into host memory.
+// Termination: the dispatcher stops when the host sets ServeCtl::stop or when
+// the host's heartbeat has not moved for idle_ticks (100 MHz s_memrealtime
+// ticks); it then sets dev_stop, and every worker ends once the item it holds
+// is past the last published one.  Every wait loop therefore ends.
+// Memory: the loop takes no acquire or release at agent or system scope: on
+// this GPU those write back or invalidate the L2 of the wave's XCD, and
+// thousands of waiting waves would keep every L2 flushing.  Instead
+//  - counters and flags are atomics (coherent at the scope they name, so
+//    every poll reads the current value);
+//  - item fields are read with system-scope loads straight from host memory
+//    (the host writes an item before its tail; a worker reads the item only
+//    after it has seen the tail);
+//  - results are written with system-scope (write-through) stores, and a wave
+//    waits for its stores to complete before it counts its item, so a
+//    request's results are in memory before its completion is counted, and
+//    the completion record's fields before its seq word.
+
+__device__ inline uint32_t lane0_u32(uint32_t v) { return __builtin_amdgcn_readlane(v, 0); }
+__device__ inline uint64_t lane0_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, 0);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), 0);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+template <class T>
+__device__ inline T sys_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <class T>
+__device__ inline T agent_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline void sys_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+// Waits until this wave's vector memory operations have completed
+// (s_waitcnt vmcnt(0); gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15): a store
+// is then acknowledged at the scope it was issued with.  No cache operation.
+__device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+template <int W>
+__device__ inline void store_track_through(const Track& t, float* __restrict__ ob) {
+  uint32_t* o = reinterpret_cast<uint32_t*>(ob);
+#pragma unroll
+  for (int e = 0; e < 6; ++e) sys_store(o + e * W, __builtin_bit_cast(uint32_t, t.p[e]));
+#pragma unroll
+  for (int e = 0; e < 21; ++e) sys_store(o + (6 + e) * W, __builtin_bit_cast(uint32_t, t.P[e]));
+  sys_store(o + 27 * W, (uint32_t)t.q);
+}
+
+__device__ void serve_dispatch(const ServeLaunch& a) {
+  uint64_t seen = ~0ull;
+  uint64_t hb = lane0_u64(sys_load(&a.ctl->heartbeat));
+  uint64_t t_hb = __builtin_amdgcn_s_memrealtime();
+  bool run = true;
+  while (run) {
+    const uint64_t t = lane0_u64(sys_load(&a.ctl->tail));
+    if (t != seen) {
+      __hip_atomic_store(&a.dev->dev_tail, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      seen = t;
+    }
+    const uint32_t stop = lane0_u32(sys_load(&a.ctl->stop));
+    const uint64_t h = lane0_u64(sys_load(&a.ctl->heartbeat));
+    const uint64_t now = __builtin_amdgcn_s_memrealtime();
+    if (h != hb) {
+      hb = h;
+      t_hb = now;
+    }
+    run = stop == 0u && now - t_hb <= a.idle_ticks;
+    if (run) __builtin_amdgcn_s_sleep(2);
+  }
+  // the final tail is stored before the stop flag, and the host learns that
+  // this launch's dispatcher has left
+  wait_stores();
+  __hip_atomic_store(&a.dev->dev_stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  sys_store(&a.ctl->exited, lane0_u32(sys_load(&a.ctl->epoch)));
+}
+
+template <int W, int kMath, int kPrefetch>
+__device__ __forceinline__ void serve_item(const ServeLaunch& a, uint64_t k, int lane) {
+  const ServeItem* it = a.items + (int64_t)(k % (uint64_t)a.item_cap);
+  const int di = W <= 64 ? lane / W : 0;
+  // the item's header and this lane's descriptor in one round trip (a
+  // padding descriptor is read too: it lies inside the record)
+  const int ndesc_v = sys_load(&it->ndesc);
+  const int half_v = sys_load(&it->half);
+  const int req_slot_v = sys_load(&it->req_slot);
+  const int req_items_v = sys_load(&it->req_items);
+  const BlockDesc* d = &it->desc[di];
+  const int64_t tb = sys_load(&d->trk_block);
+  const int64_t hblk = sys_load(&d->hit_block);
+  const int64_t ob = sys_load(&d->out_block);
+  const int nl = sys_load(&d->nlayer);
+  const int ndesc = (int)lane0_u32((uint32_t)ndesc_v);
+  const int req_slot = (int)lane0_u32((uint32_t)req_slot_v);
+  const uint32_t req_items = lane0_u32((uint32_t)req_items_v);
+  const int l = W <= 64 ? lane % W : (int)lane0_u32((uint32_t)half_v) * 64 + lane;
+  const bool valid = di < ndesc;
+  Track t{};
+  int n = 0;
+  const float* hb = a.hit;
+  if (valid) {
+    load_track<W>(t, a.trk + tb * (kTrkWords * W) + l);
+    n = a.lanes[tb * W + l];
+    n = n < nl ? n : nl;
+    n = n > 0 ? n : 0;
+    hb = a.hit + hblk * (kHitWords * W) + l;
+  }
+  run_layers_lanes<W, kMath, kPrefetch>(t, hb, n, wave_max(n));
+  if (valid) store_track_through<W>(t, a.out + ob * (kTrkWords * W) + l);
+  wait_stores();
+  if (lane == 0) {
+    const uint32_t old =
+        __hip_atomic_fetch_add(&a.done[req_slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + 1u == req_items) {
+      // the request's last item: reset its counter for the slot's next
+      // request and report it
+      __hip_atomic_store(&a.done[req_slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t c =
+          __hip_atomic_fetch_add(&a.dev->comp_tail, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ServeCompletion* r = a.comps + (int64_t)(c % (uint64_t)a.comp_cap);
+      sys_store(&r->req_slot, req_slot);
+      wait_stores();
+      sys_store(&r->seq, c + 1);
+    }
+  }
+}
+
+// Claims item indexes in order and computes them; ends once it holds an
+// index past the last published item after the dispatcher has stopped.  The
+// next index is claimed (by lane 0) while the current item starts, as in the
+// persistent ragged kernel: the claim's round trip overlaps the item, and the
+// lane-0 branch sits inside the loop body, never on the edge back to the
+// loop's head (where the compiler may split a loop whose back edges differ by
+// lane, letting the other lanes run on without lane 0 and its claim).  Every
+// loop condition is a wave-uniform value (lane 0's, read into a scalar).
+template <int W, int kMath, int kPrefetch>
+__device__ void serve_work(const ServeLaunch& a, int lane) {
+  uint64_t claimed = 0;
+  if (lane == 0)
+    claimed = __hip_atomic_fetch_add(&a.dev->head, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t k = lane0_u64(claimed);
+  bool have = true;
+  while (have) {
+    bool wait = true;
+    while (wait) {
+      if (k < lane0_u64(agent_load(&a.dev->dev_tail))) {
+        wait = false;
+      } else if (lane0_u32(agent_load(&a.dev->dev_stop))) {
+        // stop comes after the final tail: an item past it is never published
+        have = k < lane0_u64(agent_load(&a.dev->dev_tail));
+        wait = false;
+      } else {
+        __builtin_amdgcn_s_sleep(32);
+      }
+    }
+    if (have) {
+      uint64_t next = 0;
+      if (lane == 0)
+        next = __hip_atomic_fetch_add(&a.dev->head, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      serve_item<W, kMath, kPrefetch>(a, k, lane);
+      k = lane0_u64(next);
+    }
+  }
+}
+
+template <int W, int kMath, int kPrefetch>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMath == kFp64Gain ? 4 : 1))) void
+k_serve_resident(ServeLaunch a) {
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    serve_dispatch(a);
+    return;
+  }
+  serve_work<W, kMath, kPrefetch>(a, threadIdx.x & 63);
+}
+
 __global__ void k_reset_counter(unsigned* counter) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
 }
@@ -593,6 +781,21 @@ hipError_t launch_ragged(const float* trk, const float* hit, float* out, const i
       else
         hipLaunchKernelGGL((k_ragged_persistent<W, M, 1>), dim3((unsigned)g), dim3(tpw), 0, s, trk, hit, out,
                            lane_nlay, blocks, nblocks, counter, claim);
+      return hipGetLastError();
+    });
+  });
+}
+
+hipError_t launch_serve_resident(const ServeLaunch& a, int width, int math, int prefetch, int tpw, int64_t grid,
+                                 hipStream_t s) {
+  if (grid < 2 && tpw <= 64) return hipErrorInvalidValue;  // the dispatcher needs a worker beside it
+  return dispatch_width(width, [&](auto wc) {
+    return dispatch_math(math, [&](auto mc) {
+      constexpr int W = decltype(wc)::value, M = decltype(mc)::value;
+      if (prefetch >= 2)
+        hipLaunchKernelGGL((k_serve_resident<W, M, 2>), dim3((unsigned)grid), dim3(tpw), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_serve_resident<W, M, 1>), dim3((unsigned)grid), dim3(tpw), 0, s, a);
       return hipGetLastError();
     });
   });

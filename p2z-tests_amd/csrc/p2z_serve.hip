@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <cmath>
 #include <deque>
 #include <map>
@@ -67,6 +68,8 @@ struct Request {
   int64_t out_block = 0, nblocks = 0;
   double t_arrival = 0, t_admit = 0, t_launch = 0, t_done = 0;
   bool done = false;
+  int32_t slot = -1;        // resident loop: completion slot
+  uint64_t first_item = 0;  // resident loop: first ring item
 };
 
 struct Slot {
@@ -78,6 +81,34 @@ struct Slot {
   int64_t cap = 0;
   std::vector<int64_t> requests;
   bool busy = false;
+};
+
+// The device-resident loop (P2Z_SERVE_RESIDENT): host rings, device state and
+// the bookkeeping of what may be overwritten.
+constexpr int64_t kResItems = 65536;  // work-item ring (9 MiB of pinned memory)
+constexpr int32_t kResSlots = 4096;   // requests in flight; also the completion ring
+constexpr double kResIdleSeconds = 2.0;
+
+struct Resident {
+  hipStream_t stream = nullptr;
+  p2z::ServeItem* items_h = nullptr;  // pinned, coherent
+  p2z::ServeItem* items_d = nullptr;  // its device view
+  p2z::ServeCtl* ctl_h = nullptr;
+  p2z::ServeCtl* ctl_d = nullptr;
+  p2z::ServeCompletion* comps_h = nullptr;
+  p2z::ServeCompletion* comps_d = nullptr;
+  p2z::ServeDev* dev = nullptr;       // device memory: ServeDev + done[kResSlots]
+  uint64_t published = 0;             // items published (ServeCtl::tail)
+  uint64_t comp_read = 0;             // completion records consumed
+  uint64_t heartbeat = 0;
+  std::vector<int64_t> slot_req;      // request of each slot (-1: free)
+  std::vector<int32_t> free_slots;
+  std::map<uint64_t, int64_t> unfinished;  // first item -> request, in publication order
+  bool running = false;
+  uint32_t epoch = 0;  // launches so far (ServeCtl::epoch of the current one)
+  int64_t launches = 0;
+  int tpw = 256, prefetch = 2;
+  int64_t grid = 0;
 };
 
 double now_s() {
@@ -99,6 +130,7 @@ struct p2z_server {
   std::vector<Slot> slots;
   std::vector<p2z_completion> ready;
   int64_t batches = 0, batch_events = 0;
+  Resident* res = nullptr;  // non-null: the device-resident loop serves
 };
 
 namespace {
@@ -191,6 +223,164 @@ int launch_batches(p2z_server* sv) {
   return P2Z_OK;
 }
 
+// ---- device-resident loop ----------------------------------------------------
+
+template <class T>
+void host_store(T* p, T v) {
+  __atomic_store_n(p, v, __ATOMIC_RELEASE);
+}
+template <class T>
+T host_load(const T* p) {
+  return __atomic_load_n(p, __ATOMIC_ACQUIRE);
+}
+
+// Launches the loop on the server's stream.  A relaunch (the previous loop
+// ended on the idle timeout) resumes at the first item the previous loop did
+// not claim before it ended: it computed every item below its final tail and
+// none at or above it.
+int resident_launch(p2z_server* sv) {
+  Resident& R = *sv->res;
+  const p2z_server_opts& o = sv->opts;
+  P2Z_HIP_CHECK(hipMemcpyAsync(&R.dev->head, &R.dev->dev_tail, sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                               R.stream));
+  P2Z_HIP_CHECK(hipMemsetAsync(&R.dev->dev_stop, 0, sizeof(uint32_t), R.stream));
+  host_store(&R.ctl_h->stop, 0u);
+  host_store(&R.ctl_h->epoch, ++R.epoch);
+  host_store(&R.ctl_h->heartbeat, ++R.heartbeat);
+  p2z::ServeLaunch a{};
+  a.trk = sv->trk;
+  a.hit = sv->hit;
+  a.out = sv->arena_mem;
+  a.lanes = sv->lanes;
+  a.items = R.items_d;
+  a.item_cap = kResItems;
+  a.ctl = R.ctl_d;
+  a.comps = R.comps_d;
+  a.comp_cap = kResSlots;
+  a.dev = R.dev;
+  a.done = reinterpret_cast<uint32_t*>(R.dev + 1);
+  a.idle_ticks = (uint64_t)(kResIdleSeconds * 1e8);
+  P2Z_HIP_CHECK(p2z::launch_serve_resident(a, o.width, o.math, R.prefetch, R.tpw, R.grid, R.stream));
+  R.running = true;
+  R.launches += 1;
+  return P2Z_OK;
+}
+
+// Keeps the loop alive: heartbeat, and a relaunch when its dispatcher has left
+// (idle timeout) while requests wait.  The relaunch queues on the loop's
+// stream behind the kernel that is ending, whose workers still finish every
+// item below its final tail.
+int resident_keepalive(p2z_server* sv) {
+  Resident& R = *sv->res;
+  host_store(&R.ctl_h->heartbeat, ++R.heartbeat);
+  if (R.running && host_load(&R.ctl_h->exited) == R.epoch) R.running = false;
+  if (!R.running && !R.unfinished.empty()) P2Z_TRY(resident_launch(sv));
+  return P2Z_OK;
+}
+
+// Collects the completion records the loop has written.
+int resident_harvest(p2z_server* sv) {
+  Resident& R = *sv->res;
+  for (;;) {
+    p2z::ServeCompletion& c = R.comps_h[R.comp_read % kResSlots];
+    if (host_load(&c.seq) != R.comp_read + 1) break;
+    const int32_t slot = host_load(&c.req_slot);
+    if (slot < 0 || slot >= kResSlots || R.slot_req[slot] < 0)
+      return p2z::fail(P2Z_ESTATE, "completion record %llu names slot %d, which has no request",
+                       (unsigned long long)R.comp_read, slot);
+    const int64_t id = R.slot_req[slot];
+    Request& r = sv->reqs.at(id);
+    r.done = true;
+    r.t_done = now_s();
+    sv->ready.push_back(p2z_completion{id, r.out_block, r.nblocks, r.t_arrival, r.t_admit, r.t_launch, r.t_done});
+    R.unfinished.erase(r.first_item);
+    R.slot_req[slot] = -1;
+    R.free_slots.push_back(slot);
+    R.comp_read += 1;
+  }
+  return P2Z_OK;
+}
+
+// Items a request of nblocks descriptors takes at width W.
+int64_t resident_items(int64_t nblocks, int W) {
+  return W <= 64 ? (nblocks * W + 63) / 64 : 2 * nblocks;
+}
+
+// Publishes an admitted request: its descriptors longest-first, packed into
+// work items in the ring, then one release store of the new tail.
+void resident_publish(p2z_server* sv, int64_t id, Request& r, int32_t slot) {
+  Resident& R = *sv->res;
+  const int W = sv->opts.width;
+  std::vector<p2z_block_desc> d = r.blocks;
+  std::stable_sort(d.begin(), d.end(),
+                   [](const p2z_block_desc& a, const p2z_block_desc& b) { return a.nlayer > b.nlayer; });
+  const int64_t n = resident_items(r.nblocks, W);
+  const int per = W <= 64 ? 64 / W : 1;
+  for (int64_t i = 0; i < n; ++i) {
+    p2z::ServeItem& it = R.items_h[(R.published + i) % kResItems];
+    const int64_t first = W <= 64 ? i * per : i / 2;
+    const int nd = (int)std::min<int64_t>(per, r.nblocks - first);
+    for (int k = 0; k < nd; ++k) {
+      const p2z_block_desc& s = d[first + k];
+      it.desc[k] = p2z::BlockDesc{s.trk_block, s.hit_block, s.out_block, s.nlayer, s.event};
+    }
+    it.ndesc = nd;
+    it.half = W <= 64 ? 0 : (int32_t)(i % 2);
+    it.req_slot = slot;
+    it.req_items = (int32_t)n;
+  }
+  r.slot = slot;
+  r.first_item = R.published;
+  R.slot_req[slot] = id;
+  R.unfinished.emplace(R.published, id);
+  R.published += (uint64_t)n;
+  host_store(&R.ctl_h->tail, R.published);
+  r.t_launch = now_s();
+}
+
+int resident_create(p2z_server* sv) {
+  auto* R = new Resident;
+  sv->res = R;
+  const p2z_server_opts& o = sv->opts;
+  R->tpw = o.tracks_per_wg ? o.tracks_per_wg : sv->ctx->cfg.tracks_per_wg;
+  R->prefetch = o.prefetch ? o.prefetch : 2;
+  const int wpc = o.waves_per_cu ? o.waves_per_cu : 8;
+  R->grid = std::max<int64_t>(2, ((int64_t)sv->ctx->cus * wpc * 64 + R->tpw - 1) / R->tpw);
+  R->slot_req.assign(kResSlots, -1);
+  for (int32_t i = kResSlots - 1; i >= 0; --i) R->free_slots.push_back(i);
+  P2Z_HIP_CHECK(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
+  const unsigned fl = hipHostMallocCoherent | hipHostMallocMapped;
+  P2Z_HIP_CHECK(hipHostMalloc((void**)&R->items_h, kResItems * sizeof(p2z::ServeItem), fl));
+  P2Z_HIP_CHECK(hipHostMalloc((void**)&R->ctl_h, sizeof(p2z::ServeCtl), fl));
+  P2Z_HIP_CHECK(hipHostMalloc((void**)&R->comps_h, kResSlots * sizeof(p2z::ServeCompletion), fl));
+  std::memset(R->items_h, 0, kResItems * sizeof(p2z::ServeItem));
+  std::memset(R->ctl_h, 0, sizeof(p2z::ServeCtl));
+  std::memset(R->comps_h, 0, kResSlots * sizeof(p2z::ServeCompletion));
+  P2Z_HIP_CHECK(hipHostGetDevicePointer((void**)&R->items_d, R->items_h, 0));
+  P2Z_HIP_CHECK(hipHostGetDevicePointer((void**)&R->ctl_d, R->ctl_h, 0));
+  P2Z_HIP_CHECK(hipHostGetDevicePointer((void**)&R->comps_d, R->comps_h, 0));
+  const size_t dev_bytes = sizeof(p2z::ServeDev) + kResSlots * sizeof(uint32_t);
+  P2Z_HIP_CHECK(hipMalloc((void**)&R->dev, dev_bytes));
+  P2Z_HIP_CHECK(hipMemsetAsync(R->dev, 0, dev_bytes, R->stream));
+  P2Z_TRY(resident_launch(sv));
+  return P2Z_OK;
+}
+
+// Ends the loop (after the published items) and frees its memory.
+void resident_destroy(p2z_server* sv) {
+  Resident* R = sv->res;
+  if (!R) return;
+  if (R->ctl_h) host_store(&R->ctl_h->stop, 1u);
+  if (R->stream) (void)hipStreamSynchronize(R->stream);
+  if (R->dev) (void)hipFree(R->dev);
+  if (R->items_h) (void)hipHostFree(R->items_h);
+  if (R->ctl_h) (void)hipHostFree(R->ctl_h);
+  if (R->comps_h) (void)hipHostFree(R->comps_h);
+  if (R->stream) (void)hipStreamDestroy(R->stream);
+  delete R;
+  sv->res = nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -214,6 +404,7 @@ static int server_create_impl(p2z_ctx* ctx, const void* trk, const void* hit, co
   if (o.items_per_claim < 0 || o.items_per_claim > 64) return p2z::fail(P2Z_EINVAL, "items_per_claim in [0,64]");
   if (o.pool_blocks <= 0 || o.pool_hit_blocks <= 0)
     return p2z::fail(P2Z_EINVAL, "pool_blocks and pool_hit_blocks (the pool's sizes) are required");
+  if (o.flags & ~P2Z_SERVE_RESIDENT) return p2z::fail(P2Z_EINVAL, "unknown server flags 0x%x", o.flags);
   const int64_t blk_bytes = (int64_t)p2z::kTrkWords * o.width * sizeof(float);
   const int64_t nblk = o.budget_bytes / blk_bytes;
   if (nblk <= 0) return p2z::fail(P2Z_EINVAL, "budget of %lld bytes holds no result block", (long long)o.budget_bytes);
@@ -229,8 +420,10 @@ static int server_create_impl(p2z_ctx* ctx, const void* trk, const void* hit, co
   const hipError_t e = hipMalloc(&sv->arena_mem, nblk * blk_bytes);
   if (e == hipErrorOutOfMemory) rc = p2z::fail(P2Z_ENOMEM, "arena of %lld bytes does not fit", (long long)(nblk * blk_bytes));
   else rc = p2z::hip_status(e, "hipMalloc");
-  if (rc == P2Z_OK) {
-    sv->arena = new Arena(nblk);
+  if (rc == P2Z_OK) sv->arena = new Arena(nblk);
+  if (rc == P2Z_OK && (o.flags & P2Z_SERVE_RESIDENT)) {
+    rc = resident_create(sv);
+  } else if (rc == P2Z_OK) {
     sv->slots.resize(o.nstreams);
     for (auto& s : sv->slots) {
       if (rc == P2Z_OK) rc = p2z::hip_status(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -253,6 +446,7 @@ int p2z_server_create(p2z_ctx* ctx, const void* trk, const void* hit, const int3
 int p2z_server_destroy(p2z_server* sv) {
   if (!sv) return P2Z_OK;
   if (sv->ctx) (void)p2z::use_device(sv->ctx);
+  resident_destroy(sv);
   for (auto& s : sv->slots) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (s.d_desc) (void)hipFree(s.d_desc);
@@ -277,6 +471,20 @@ static int server_submit_impl(p2z_server* sv, int64_t request, const p2z_block_d
                      (long long)nblocks, (long long)sv->arena->total());
   if (p2z_check_blocks(blocks, nblocks, sv->opts.pool_blocks, sv->opts.pool_hit_blocks, nblocks) != P2Z_OK)
     return p2z::fail(P2Z_EINVAL, "request %lld: %s", (long long)request, p2z_last_error());
+  if (sv->res) {
+    Resident& R = *sv->res;
+    const int64_t n = resident_items(nblocks, sv->opts.width);
+    if (n > kResItems)
+      return p2z::fail(P2Z_EINVAL, "request %lld needs %lld work items, the ring holds %lld", (long long)request,
+                       (long long)n, (long long)kResItems);
+    P2Z_TRY(p2z::use_device(sv->ctx));
+    P2Z_TRY(resident_harvest(sv));
+    const uint64_t oldest = R.unfinished.empty() ? R.published : R.unfinished.begin()->first;
+    if (R.free_slots.empty() || R.published + (uint64_t)n - oldest > (uint64_t)kResItems)
+      return p2z::fail(P2Z_EBUSY, "resident loop full (%lld requests, %llu items in flight)",
+                       (long long)(kResSlots - (int64_t)R.free_slots.size()),
+                       (unsigned long long)(R.published - oldest));
+  }
   const int64_t at = sv->arena->alloc(nblocks);
   if (at < 0) return p2z::fail(P2Z_EBUSY, "arena full (%lld of %lld blocks in use)", (long long)sv->arena->used(),
                                (long long)sv->arena->total());
@@ -287,6 +495,14 @@ static int server_submit_impl(p2z_server* sv, int64_t request, const p2z_block_d
   r.nblocks = nblocks;
   r.t_arrival = t_arrival;
   r.t_admit = now_s();
+  if (sv->res) {
+    Resident& R = *sv->res;
+    const int32_t slot = R.free_slots.back();
+    R.free_slots.pop_back();
+    Request& rr = sv->reqs.emplace(request, std::move(r)).first->second;
+    resident_publish(sv, request, rr, slot);
+    return resident_keepalive(sv);
+  }
   sv->reqs.emplace(request, std::move(r));
   sv->pending.push_back(request);
   return P2Z_OK;
@@ -301,6 +517,12 @@ int p2z_server_submit(p2z_server* sv, int64_t request, const p2z_block_desc* blo
 static int server_pump_impl(p2z_server* sv) {
   if (!sv) return p2z::fail(P2Z_EINVAL, "null server");
   P2Z_TRY(p2z::use_device(sv->ctx));
+  if (sv->res) {
+    P2Z_TRY(resident_harvest(sv));
+    P2Z_TRY(resident_keepalive(sv));
+    sv->batches = sv->res->launches;
+    return (int)sv->ready.size();
+  }
   P2Z_TRY(harvest(sv));
   P2Z_TRY(launch_batches(sv));
   return (int)sv->ready.size();
@@ -360,15 +582,25 @@ static int serve_poisson_impl(p2z_server* sv, const p2z_block_desc* blocks, cons
   }
   std::deque<int64_t> backlog;
   int64_t next = 0, ndone = 0, inflight = 0, max_inflight = 0, rejections = 0;
-  double t_last = t0;
+  double t_last = t0, t_submit = 0, t_pump = 0;
   std::vector<p2z_completion> comp(256);
   sv->batches = sv->batch_events = 0;
+  int64_t launches0 = 0;
+  if (sv->res) {
+    // the loop that serves the run counts as one launch, relaunches as more
+    P2Z_TRY(p2z::use_device(sv->ctx));
+    P2Z_TRY(resident_keepalive(sv));
+    if (!sv->res->running) P2Z_TRY(resident_launch(sv));
+    launches0 = sv->res->launches - 1;
+  }
   while (ndone < nevents) {
     const double now = now_s();
     while (next < nevents && arrive[next] <= now) backlog.push_back(next++);
     while (!backlog.empty()) {
       const int64_t e = backlog.front();
+      const double ts = now_s();
       const int rc = p2z_server_submit(sv, e, ev[e].data(), (int64_t)ev[e].size(), arrive[e]);
+      t_submit += now_s() - ts;
       if (rc == P2Z_EBUSY) {
         // nothing in flight and still no room: the kept results fill the arena
         if (inflight == 0 && !release)
@@ -380,7 +612,9 @@ static int serve_poisson_impl(p2z_server* sv, const p2z_block_desc* blocks, cons
       backlog.pop_front();
       max_inflight = std::max(max_inflight, ++inflight);
     }
+    const double tp = now_s();
     const int r = p2z_server_pump(sv);
+    t_pump += now_s() - tp;
     if (r < 0) return r;
     int n;
     while ((n = p2z_server_poll(sv, comp.data(), (int32_t)comp.size())) > 0) {
@@ -405,7 +639,7 @@ static int serve_poisson_impl(p2z_server* sv, const p2z_block_desc* blocks, cons
   rep->nevents = nevents;
   rep->track_layers = 0;  // the lane layer counts live on the device: the caller fills these two
   rep->tl_per_s = 0;
-  rep->batches = sv->batches;
+  rep->batches = sv->res ? sv->res->launches - launches0 : sv->batches;
   rep->max_inflight_events = max_inflight;
   rep->busy_rejections = rejections;
   rep->rate_offered = rate;
@@ -416,7 +650,10 @@ static int serve_poisson_impl(p2z_server* sv, const p2z_block_desc* blocks, cons
   rep->lat_p999 = q(0.999);
   rep->lat_mean = mean;
   rep->lat_max = s.back();
-  rep->mean_batch_events = sv->batches ? (double)sv->batch_events / sv->batches : 0.0;
+  rep->mean_batch_events = sv->res ? (double)nevents / (double)rep->batches
+                                   : (sv->batches ? (double)sv->batch_events / sv->batches : 0.0);
+  rep->host_submit_s = t_submit;
+  rep->host_pump_s = t_pump;
   if (latency)
     for (int64_t e = 0; e < nevents; ++e) latency[e] = lat[e];
   return P2Z_OK;

@@ -93,7 +93,7 @@ class ServerOpts(C.Structure):
     _fields_ = [("budget_bytes", C.c_int64), ("nstreams", C.c_int32), ("max_batch_blocks", C.c_int32),
                 ("width", C.c_int32), ("math", C.c_int32), ("tracks_per_wg", C.c_int32), ("prefetch", C.c_int32),
                 ("waves_per_cu", C.c_int32), ("items_per_claim", C.c_int32), ("pool_blocks", C.c_int64),
-                ("pool_hit_blocks", C.c_int64)]
+                ("pool_hit_blocks", C.c_int64), ("flags", C.c_uint32), ("reserved", C.c_int32)]
 
 
 class Completion(C.Structure):
@@ -105,7 +105,8 @@ class ServeReport(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("nevents", "track_layers", "batches", "max_inflight_events",
                                           "busy_rejections")] + \
                [(n, C.c_double) for n in ("rate_offered", "rate_achieved", "t_span", "tl_per_s", "lat_p50", "lat_p99",
-                                          "lat_p999", "lat_mean", "lat_max", "mean_batch_events")]
+                                          "lat_p999", "lat_mean", "lat_max", "mean_batch_events",
+                                          "host_submit_s", "host_pump_s")]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -113,6 +114,7 @@ class ServeReport(C.Structure):
 
 RAGGED_PERSISTENT, RAGGED_GRAPH = 1, 2
 V1_FIXED = 1
+SERVE_RESIDENT = 1
 WORKLOAD = {"precision": 2, "ragged": 3}
 
 _lib = None
@@ -409,16 +411,19 @@ class DeviceBuffer:
 
 class Server:
     """Continuous-batching server over a device-resident ragged pool
-    (p2z_server_*): requests are events given by their block descriptors."""
+    (p2z_server_*): requests are events given by their block descriptors.
+    resident=True serves them with the device-resident loop (one persistent
+    kernel fed through a ring in pinned host memory) instead of batches of
+    launches."""
 
     def __init__(self, ctx: "Context", trk, hit, lane_nlayer, budget_bytes: int, width: int, math="fast",
                  nstreams: int = 0, max_batch_blocks: int = 0, tracks_per_wg: int = 0, prefetch: int = 0,
-                 waves_per_cu: int = 0, items_per_claim: int = 0):
+                 waves_per_cu: int = 0, items_per_claim: int = 0, resident: bool = False):
         self.ctx = ctx
         self.width = width
         o = ServerOpts(budget_bytes, nstreams, max_batch_blocks, width, MATH.get(math, math), tracks_per_wg,
                        prefetch, waves_per_cu, items_per_claim, trk.nbytes // (TRK_WORDS * width * 4),
-                       hit.nbytes // (HIT_WORDS * width * 4))
+                       hit.nbytes // (HIT_WORDS * width * 4), SERVE_RESIDENT if resident else 0, 0)
         self.handle = vp()
         check(lib().p2z_server_create(ctx.handle, _ptr(trk), _ptr(hit), _ptr(lane_nlayer), C.byref(o),
                                       C.byref(self.handle)), "p2z_server_create")

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     L = C.CDLL(p2z.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert not missing, missing
-    assert p2z.lib().p2z_version() == 1
+    assert p2z.lib().p2z_version() == 2
 
 
 def test_binding_covers_header():

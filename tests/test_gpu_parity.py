@@ -446,14 +446,23 @@ def test_edge_sizes(ctx):
 
 
 def test_invalid_arguments_do_not_launch(ctx):
+    """The C ABI rejects a bad width or mode before any launch (raw pointers
+    here, so that the call reaches it); the Python binding rejects a buffer
+    too small for the sizes asked before it calls the library."""
     buf = ctx.alloc(4096)
     with pytest.raises(p2z.P2zError) as e:
-        ctx.run(buf, buf, buf, 1, 1, 20, 48)
+        ctx.run(buf.ptr, buf.ptr, buf.ptr, 1, 1, 20, 48)
     assert e.value.rc == p2z.EINVAL
     with pytest.raises(p2z.P2zError):
-        ctx.run(buf, buf, buf, 1, 1, 20, 32, math=7)
+        ctx.run(buf.ptr, buf.ptr, buf.ptr, 1, 1, 20, 32, math=7)
     with pytest.raises(p2z.P2zError):
         p2z.Context(0, tracks_per_wg=100)
+    with pytest.raises(ValueError):
+        ctx.run(buf, buf, buf, 1, 1, 20, 32)           # 1 block x 20 layers needs 23 KiB of hits
+    with pytest.raises(ValueError):
+        ctx.propagate_to_z(buf, buf, buf, 2, 32)        # 2 track blocks need 7 KiB
+    with pytest.raises(ValueError):
+        ctx.run_ragged(buf, buf, buf, buf, buf, 200, 32)   # 200 descriptors need 6400 bytes
     buf.free()
 
 

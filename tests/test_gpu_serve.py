@@ -1,7 +1,10 @@
-"""GPU tests of the serving loop (p2z_server_*): results are the same alone
-and in the mix, admission by the HBM arena, request bookkeeping, and the
-Poisson driver's report.  Failure conditions are forced from the host
-(budgets, ids); every kernel launched runs to completion normally."""
+"""GPU tests of the serving loop (p2z_server_*), in both of its forms: batches
+of persistent launches, and the device-resident loop (one persistent kernel
+fed through a ring in host memory).  Results are the same alone and in the
+mix, admission by the HBM arena, request bookkeeping, and the Poisson
+driver's report.  Failure conditions are forced from the host (budgets, ids);
+every kernel launched runs to completion normally (the resident loop ends on
+the server's stop flag or on its idle timeout)."""
 from __future__ import annotations
 
 import numpy as np
@@ -10,6 +13,7 @@ import pytest
 import p2z
 
 pytestmark = pytest.mark.gpu
+FORMS = pytest.mark.parametrize("resident", [False, True], ids=["batches", "resident"])
 
 
 def log(msg: str) -> None:
@@ -36,11 +40,12 @@ def blk_bytes(W):
     return 28 * W * 4
 
 
-def test_result_is_the_same_alone_and_in_the_mix(ctx, pool):
+@FORMS
+def test_result_is_the_same_alone_and_in_the_mix(ctx, pool, resident):
     w = pool["w"]
     W = w.width
     budget = (w.nblocks + 16) * blk_bytes(W)
-    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], budget, W, "fast", nstreams=4) as sv:
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], budget, W, "fast", nstreams=4, resident=resident) as sv:
         # mixed: Poisson arrivals fast enough that several events share batches and overlap
         rep, lat, ob = sv.poisson(w.blocks, w.event_blocks, rate=50000.0, seed=3, release=False)
         log(f"  mix: {rep.batches} batches, {rep.mean_batch_events:.2f} events/batch, "
@@ -60,10 +65,11 @@ def test_result_is_the_same_alone_and_in_the_mix(ctx, pool):
             alone = sv.result(c)
             sv.release(rid)
             assert np.array_equal(alone.view(np.uint32), mixed[e].view(np.uint32)), e
-    assert rep.batches < w.nevents   # the mix really batched events together
+    assert rep.batches < w.nevents   # the mix really batched events together (resident: one loop)
 
 
-def test_served_in_reference_typing_equals_the_oracle(ctx, pool):
+@FORMS
+def test_served_in_reference_typing_equals_the_oracle(ctx, pool, resident):
     """Reference typing through the server (Poisson arrivals, events batched
     and overlapping): every served event equals the oracle's ragged driver on
     that event's blocks, word for word."""
@@ -74,7 +80,7 @@ def test_served_in_reference_typing_equals_the_oracle(ctx, pool):
     exp = O.run_ragged(w.tracks, w.hits, w.nblocks, W, b["trk_block"], b["hit_block"], b["nlayer"],
                        w.lane_nlayer)
     nb = np.diff(w.event_blocks)
-    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 64 * 1024 * 1024, W, "ref") as sv:
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 64 * 1024 * 1024, W, "ref", resident=resident) as sv:
         rep, lat, ob = sv.poisson(w.blocks, w.event_blocks, rate=20000.0, seed=5, release=False)
         log(f"  ref served: {rep.batches} batches for {w.nevents} events")
         differ = []
@@ -88,7 +94,8 @@ def test_served_in_reference_typing_equals_the_oracle(ctx, pool):
         assert not differ
 
 
-def test_matches_the_batch_path(ctx, pool):
+@FORMS
+def test_matches_the_batch_path(ctx, pool, resident):
     """A served event equals the same event in one ragged batch launch."""
     w = pool["w"]
     W = w.width
@@ -98,7 +105,7 @@ def test_matches_the_batch_path(ctx, pool):
     ref = ctx.download(out)
     blk.free()
     out.free()
-    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 64 * 1024 * 1024, W, "fast") as sv:
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 64 * 1024 * 1024, W, "fast", resident=resident) as sv:
         for e in (0, 7, w.nevents - 1):
             assert sv.submit(e, event_blocks(w, e)) == p2z.OK
             c = sv.wait([e])[e]
@@ -108,13 +115,14 @@ def test_matches_the_batch_path(ctx, pool):
             sv.release(e)
 
 
-def test_admission_is_bounded_by_the_arena(ctx, pool):
+@FORMS
+def test_admission_is_bounded_by_the_arena(ctx, pool, resident):
     w = pool["w"]
     W = w.width
     nb = np.diff(w.event_blocks)
     e0, e1 = 0, 1
     budget = int(max(nb[e0], nb[e1])) * blk_bytes(W)          # room for one of the two at a time
-    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], budget, W, "fast", nstreams=2) as sv:
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], budget, W, "fast", nstreams=2, resident=resident) as sv:
         assert sv.submit(e0, event_blocks(w, e0)) == p2z.OK
         assert sv.submit(e1, event_blocks(w, e1)) == p2z.EBUSY      # no room until e0 is released
         sv.wait([e0])
@@ -126,15 +134,16 @@ def test_admission_is_bounded_by_the_arena(ctx, pool):
     # an event larger than the whole arena is rejected outright
     small = int(nb.max() - 1) * blk_bytes(W)
     big = int(np.argmax(nb))
-    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], small, W, "fast") as sv:
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], small, W, "fast", resident=resident) as sv:
         with pytest.raises(p2z.P2zError) as err:
             sv.submit(big, event_blocks(w, big))
         assert err.value.rc == p2z.EINVAL
 
 
-def test_request_bookkeeping_errors(ctx, pool):
+@FORMS
+def test_request_bookkeeping_errors(ctx, pool, resident):
     w = pool["w"]
-    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 64 * 1024 * 1024, w.width, "fast") as sv:
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 64 * 1024 * 1024, w.width, "fast", resident=resident) as sv:
         assert sv.submit(5, event_blocks(w, 5)) == p2z.OK
         with pytest.raises(p2z.P2zError) as err:
             sv.submit(5, event_blocks(w, 5))                  # duplicate id
@@ -155,24 +164,26 @@ def test_request_bookkeeping_errors(ctx, pool):
         sv.wait([5])
         sv.release(5)
     with pytest.raises(p2z.P2zError):                         # budget below one block
-        p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 100, w.width, "fast")
+        p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 100, w.width, "fast", resident=resident)
 
 
-def test_kept_results_overflowing_the_arena_fail_cleanly(ctx, pool):
+@FORMS
+def test_kept_results_overflowing_the_arena_fail_cleanly(ctx, pool, resident):
     """Poisson run keeping every result in an arena too small for all of them:
     an error once nothing is in flight, not a wait forever."""
     w = pool["w"]
     budget = int(np.diff(w.event_blocks)[:8].sum()) * blk_bytes(w.width)
-    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], budget, w.width, "fast") as sv:
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], budget, w.width, "fast", resident=resident) as sv:
         with pytest.raises(p2z.P2zError) as err:
             sv.poisson(w.blocks, w.event_blocks, rate=20000.0, release=False)
         assert err.value.rc == p2z.ENOMEM
 
 
+@FORMS
 @pytest.mark.parametrize("rate", [2000.0, 20000.0])
-def test_poisson_report(ctx, pool, rate):
+def test_poisson_report(ctx, pool, rate, resident):
     w = pool["w"]
-    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 256 * 1024 * 1024, w.width, "fp64gain") as sv:
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 256 * 1024 * 1024, w.width, "fp64gain", resident=resident) as sv:
         rep, lat, ob = sv.poisson(w.blocks, w.event_blocks, rate=rate, seed=1, release=True)
     d = rep.as_dict()
     log(f"  rate {rate}: {d}")
@@ -182,7 +193,8 @@ def test_poisson_report(ctx, pool, rate):
     assert abs(np.median(lat) - rep.lat_p50) <= 1e-3 + 0.5 * rep.lat_p50
 
 
-def test_full_size_served_equals_batch():
+@FORMS
+def test_full_size_served_equals_batch(resident):
     """The serving workload at full size: 2000 ragged events with the FP64
     gain steps, arriving at 200k events/s so that many share batches and
     overlap on the streams.  Every served event equals the same event in one
@@ -199,7 +211,7 @@ def test_full_size_served_equals_batch():
         ref = c.download(out)
         out.free()
         blk.free()
-        with p2z.Server(c, trk, hit, lanes, 64 * 2**30, W, "fp64gain", nstreams=4) as sv:
+        with p2z.Server(c, trk, hit, lanes, 64 * 2**30, W, "fp64gain", nstreams=4, resident=resident) as sv:
             rep, lat, ob = sv.poisson(w.blocks, w.event_blocks, rate=200000.0, seed=5, release=False)
             log(f"  {rep.batches} batches, {rep.mean_batch_events:.2f} events/batch, "
                 f"max in flight {rep.max_inflight_events}")
@@ -213,3 +225,22 @@ def test_full_size_served_equals_batch():
                 sv.release(e)
         for b in (trk, hit, lanes):
             b.free()
+
+
+def test_resident_loop_restarts_after_idle(ctx, pool):
+    """The resident loop ends by itself when the host neither submits nor
+    pumps for about two seconds (its bounded wait); the next call launches it
+    again, and requests are served as before, with the same bits."""
+    import time
+    w = pool["w"]
+    W = w.width
+    with p2z.Server(ctx, pool["trk"], pool["hit"], pool["lanes"], 64 * 1024 * 1024, W, "fast",
+                    resident=True) as sv:
+        assert sv.submit(1, event_blocks(w, 1)) == p2z.OK
+        first = sv.result(sv.wait([1])[1])
+        sv.release(1)
+        time.sleep(3.0)                              # the loop's idle timeout passes
+        assert sv.submit(2, event_blocks(w, 1)) == p2z.OK
+        again = sv.result(sv.wait([2])[2])
+        sv.release(2)
+    assert np.array_equal(first.view(np.uint32), again.view(np.uint32))
