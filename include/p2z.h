@@ -357,6 +357,21 @@ typedef struct {
 } p2z_summary;
 int p2z_summary_device(p2z_ctx* ctx, const void* out, int64_t nevts, int64_t nb, int32_t width,
                        int64_t ntrks, const float lasthit[3], p2z_summary* s);
+/* p2z_summary_device without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory at call time
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and result (bit for bit) as
+ * p2z_summary_device, with their scratch in a static device array of
+ * P2Z_SUMMARY_NOALLOC_BYTES per GPU (set up by p2z_create, held until the
+ * process ends) instead of context memory grown by hipMalloc.  At most
+ * P2Z_SUMMARY_NOALLOC_MAX_TRACKS tracks (nevts * ntrks), else P2Z_EINVAL.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from two threads or two contexts) race and must be serialised by the
+ * caller.  Synchronous. */
+#define P2Z_SUMMARY_NOALLOC_BYTES (4 << 20)
+#define P2Z_SUMMARY_NOALLOC_MAX_TRACKS 11184128LL /* (4 MiB / 8 - 32) / 12 * 256 */
+int p2z_summary_device_noalloc(p2z_ctx* ctx, const void* out, int64_t nevts, int64_t nb, int32_t width,
+                               int64_t ntrks, const float lasthit[3], p2z_summary* s);
 /* The same statistics on host memory (used by the drop-in report). */
 int p2z_summary_host(const void* out, int64_t nevts, int64_t nb, int32_t width, int64_t ntrks,
                      const float lasthit[3], p2z_summary* s);

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -27,6 +28,45 @@ int use_device(p2z_ctx* c) {
 }
 }  // namespace p2z
 
+namespace {
+
+// Device scratch of p2z_summary_device_noalloc: results in the first
+// kNoallocResDoubles doubles, the report kernels' per-workgroup partial sums
+// after them.  A file-scope device array: every GPU the library runs on holds
+// its own copy, at its own address, until the process ends, and every call on
+// that GPU shares it.  Host code never passes the array's name to a kernel or
+// a copy (on the host it is only a stand-in for the symbol); it uses the
+// device address hipGetSymbolAddress returns, fetched once per device when
+// the first context on that device is created.
+constexpr size_t kNoallocDoubles = (size_t)P2Z_SUMMARY_NOALLOC_BYTES / sizeof(double);
+constexpr size_t kNoallocResDoubles = 32;
+static_assert(P2Z_SUMMARY_NOALLOC_MAX_TRACKS ==
+                  (P2Z_SUMMARY_NOALLOC_BYTES / 8 - 32) / 12 * 256,
+              "maximum tracks of the header = partial slots of the scratch x 256 tracks");
+__device__ __align__(256) double scratch[kNoallocDoubles];
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // by device ordinal; null until fetched
+
+// Fetches (once per device) the device address of `scratch` on the current
+// device, which must be `device`.
+int noalloc_scratch_init(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return P2Z_OK;
+  void* ptr = nullptr;
+  P2Z_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return p2z::fail(P2Z_EHIP, "hipGetSymbolAddress returned a null scratch address");
+  g_scratch_ptr[device] = static_cast<double*>(ptr);
+  return P2Z_OK;
+}
+
+double* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+
+}  // namespace
 
 namespace {
 using p2z::use_device;
@@ -106,6 +146,7 @@ static int create_impl(p2z_ctx** out, int device, const p2z_opts* opts) {
   if (rc == P2Z_OK) rc = p2z::hip_status(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   if (rc == P2Z_OK) rc = p2z::hip_status(hipEventCreate(&c->t0), "hipEventCreate");
   if (rc == P2Z_OK) rc = p2z::hip_status(hipEventCreate(&c->t1), "hipEventCreate");
+  if (rc == P2Z_OK) rc = noalloc_scratch_init(device);
   if (rc == P2Z_OK) rc = apply_opts(c, opts);
   if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc((void**)&c->sum_res, 32 * sizeof(double)), "hipMalloc");
   if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc((void**)&c->counter, 256), "hipMalloc");
@@ -522,6 +563,20 @@ int p2z_summary_device(p2z_ctx* c, const void* out, int64_t nevts, int64_t nb, i
     c->sum_part_len = need;
   }
   return summary_run(c, out, nevts, nb, width, ntrks, lasthit, c->sum_part, c->sum_res, s);
+}
+
+int p2z_summary_device_noalloc(p2z_ctx* c, const void* out, int64_t nevts, int64_t nb, int32_t width,
+                               int64_t ntrks, const float lasthit[3], p2z_summary* s) {
+  P2Z_TRY(summary_args(c, out, nevts, nb, width, ntrks, lasthit, s));
+  if (nevts > P2Z_SUMMARY_NOALLOC_MAX_TRACKS / ntrks)
+    return p2z::fail(P2Z_EINVAL, "%lld events x %lld tracks exceed the %lld tracks of the static report scratch",
+                     (long long)nevts, (long long)ntrks, (long long)P2Z_SUMMARY_NOALLOC_MAX_TRACKS);
+  P2Z_TRY(use_device(c));
+  double* base = noalloc_scratch(c->device);
+  if (!base) return p2z::fail(P2Z_ESTATE, "static report scratch of GPU %d not set up", c->device);
+  // every partial slot the kernels read is written first in the same call,
+  // and the results are written before they are read: nothing to clear
+  return summary_run(c, out, nevts, nb, width, ntrks, lasthit, base + kNoallocResDoubles, base, s);
 }
 
 }  // extern "C"

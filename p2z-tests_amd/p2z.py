@@ -114,6 +114,7 @@ class ServeReport(C.Structure):
 
 RAGGED_PERSISTENT, RAGGED_GRAPH = 1, 2
 V1_FIXED = 1
+SUMMARY_NOALLOC_MAX_TRACKS = 11184128
 SERVE_RESIDENT = 1
 WORKLOAD = {"precision": 2, "ragged": 3}
 
@@ -174,6 +175,7 @@ _SIGS = {
     "p2z_summary_host": ([vp, i64, i64, i32, i64, C.POINTER(C.c_float), C.POINTER(Summary)], i32),
     "p2z_format_summary": ([C.POINTER(Summary), C.c_char_p, i32], i32),
     "p2z_sincosf_ref": ([vp, u32, i64, vp, vp], i32),
+    "p2z_summary_device_noalloc": ([vp, vp, i64, i64, i32, i64, C.POINTER(C.c_float), C.POINTER(Summary)], i32),
 }
 
 
@@ -660,6 +662,17 @@ class Context:
         o = RaggedOpts(tracks_per_wg, prefetch, waves_per_cu, items_per_claim)
         check(lib().p2z_run_ragged(self.handle, _ptr(trk), _ptr(hit), _ptr(out), _ptr(lane_nlayer), _ptr(blocks),
                                    nblocks, width, MATH.get(math, math), flags, C.byref(o)), "p2z_run_ragged")
+
+    def summary_device_noalloc(self, out: DeviceBuffer, nevts: int, nb: int, width: int, ntrks: int,
+                               lasthit) -> Summary:
+        """summary_device with its scratch in the library's static device
+        array (no allocation; at most SUMMARY_NOALLOC_MAX_TRACKS tracks)."""
+        _need(out, 4 * trk_words(nevts * nb, width), "out")
+        s = Summary()
+        last = (C.c_float * 3)(*[float(v) for v in lasthit])
+        check(lib().p2z_summary_device_noalloc(self.handle, _ptr(out), nevts, nb, width, ntrks, last, C.byref(s)),
+              "p2z_summary_device_noalloc")
+        return s
 
     def sincosf_ref(self, first_bits: int, n: int, s: DeviceBuffer, c: DeviceBuffer) -> None:
         """sin and cos of the reference-typing mode for the float bit patterns
