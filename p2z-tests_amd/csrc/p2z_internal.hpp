@@ -106,10 +106,14 @@ struct ServeCtl {
   uint32_t pad2[13];
 };
 
+// Item claims are striped over kServeStripes counters, one per 128-byte line:
+// workgroup g claims the items k with k % kServeStripes == g % kServeStripes,
+// so that no single counter takes every claim of the device.
+constexpr int kServeStripes = 8;
+
 // Device-side state of the loop (device memory, zeroed before the first launch).
 struct ServeDev {
-  uint64_t head;       // items claimed by workers
-  uint64_t pad0[7];
+  uint64_t head[kServeStripes][16];  // per stripe: items of the stripe claimed
   uint64_t dev_tail;   // copy of ServeCtl::tail kept by the dispatcher wave
   uint64_t pad1[7];
   uint32_t dev_stop;   // set by the dispatcher: workers end once nothing is left
@@ -134,9 +138,15 @@ struct ServeLaunch {
   uint64_t idle_ticks;       // heartbeat timeout, in s_memrealtime ticks (100 MHz)
 };
 
+// Prepares a (re)launch on stream s: every stripe's claim counter is set to
+// its first item at or past dev_tail (items below it are done), dev_stop to 0.
+hipError_t launch_serve_reset(ServeDev* dev, hipStream_t s);
+
 // One persistent launch of the resident loop (grid workgroups of tpw
-// threads; wave 0 of workgroup 0 is the dispatcher).  Returns when launched;
-// the kernel runs until ServeCtl::stop or the heartbeat timeout.
+// threads; wave 0 of workgroup 0 is the dispatcher; at least
+// 2 * kServeStripes workgroups, so that every stripe has workers).  Returns
+// when launched; the kernel runs until ServeCtl::stop or the heartbeat
+// timeout.
 hipError_t launch_serve_resident(const ServeLaunch& a, int width, int math, int prefetch, int tpw, int64_t grid,
                                  hipStream_t s);
 

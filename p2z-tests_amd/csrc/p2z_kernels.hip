@@ -372,8 +372,11 @@ __device__ __forceinline__ void serve_item(const ServeLaunch& a, uint64_t k, int
   }
 }
 
-// Claims item indexes in order and computes them; ends once it holds an
-// index past the last published item after the dispatcher has stopped.  The
+// Claims the item indexes of its workgroup's stripe in order and computes
+// them; ends once it holds an index past the last published item after the
+// dispatcher has stopped.  Every stripe has workers (the grid has at least two
+// workgroups per stripe, dispatched in order), so every published item is
+// claimed.  The
 // next index is claimed (by lane 0) while the current item starts, as in the
 // persistent ragged kernel: the claim's round trip overlaps the item, and the
 // lane-0 branch sits inside the loop body, never on the edge back to the
@@ -382,10 +385,11 @@ __device__ __forceinline__ void serve_item(const ServeLaunch& a, uint64_t k, int
 // loop condition is a wave-uniform value (lane 0's, read into a scalar).
 template <int W, int kMath, int kPrefetch>
 __device__ void serve_work(const ServeLaunch& a, int lane) {
+  const uint64_t stripe = blockIdx.x % kServeStripes;
+  uint64_t* head = &a.dev->head[stripe][0];
   uint64_t claimed = 0;
-  if (lane == 0)
-    claimed = __hip_atomic_fetch_add(&a.dev->head, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t k = lane0_u64(claimed);
+  if (lane == 0) claimed = __hip_atomic_fetch_add(head, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t k = lane0_u64(claimed) * kServeStripes + stripe;
   bool have = true;
   while (have) {
     bool wait = true;
@@ -402,12 +406,18 @@ __device__ void serve_work(const ServeLaunch& a, int lane) {
     }
     if (have) {
       uint64_t next = 0;
-      if (lane == 0)
-        next = __hip_atomic_fetch_add(&a.dev->head, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) next = __hip_atomic_fetch_add(head, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       serve_item<W, kMath, kPrefetch>(a, k, lane);
-      k = lane0_u64(next);
+      k = lane0_u64(next) * kServeStripes + stripe;
     }
   }
+}
+
+__global__ void k_serve_reset(ServeDev* dev) {
+  const uint64_t t = dev->dev_tail;
+  const int j = threadIdx.x;
+  if (j < kServeStripes) dev->head[j][0] = t > (uint64_t)j ? (t - j + kServeStripes - 1) / kServeStripes : 0;
+  if (j == 0) dev->dev_stop = 0u;
 }
 
 template <int W, int kMath, int kPrefetch>
@@ -786,9 +796,14 @@ hipError_t launch_ragged(const float* trk, const float* hit, float* out, const i
   });
 }
 
+hipError_t launch_serve_reset(ServeDev* dev, hipStream_t s) {
+  hipLaunchKernelGGL(k_serve_reset, dim3(1), dim3(64), 0, s, dev);
+  return hipGetLastError();
+}
+
 hipError_t launch_serve_resident(const ServeLaunch& a, int width, int math, int prefetch, int tpw, int64_t grid,
                                  hipStream_t s) {
-  if (grid < 2 && tpw <= 64) return hipErrorInvalidValue;  // the dispatcher needs a worker beside it
+  if (grid < 2 * kServeStripes) return hipErrorInvalidValue;  // every stripe needs workers
   return dispatch_width(width, [&](auto wc) {
     return dispatch_math(math, [&](auto mc) {
       constexpr int W = decltype(wc)::value, M = decltype(mc)::value;

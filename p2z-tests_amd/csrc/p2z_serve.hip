@@ -236,14 +236,12 @@ T host_load(const T* p) {
 
 // Launches the loop on the server's stream.  A relaunch (the previous loop
 // ended on the idle timeout) resumes at the first item the previous loop did
-// not claim before it ended: it computed every item below its final tail and
-// none at or above it.
+// not compute: it computed every item below its final tail and none at or
+// above it (the reset kernel, queued behind it, sets the claim counters).
 int resident_launch(p2z_server* sv) {
   Resident& R = *sv->res;
   const p2z_server_opts& o = sv->opts;
-  P2Z_HIP_CHECK(hipMemcpyAsync(&R.dev->head, &R.dev->dev_tail, sizeof(uint64_t), hipMemcpyDeviceToDevice,
-                               R.stream));
-  P2Z_HIP_CHECK(hipMemsetAsync(&R.dev->dev_stop, 0, sizeof(uint32_t), R.stream));
+  P2Z_HIP_CHECK(p2z::launch_serve_reset(R.dev, R.stream));
   host_store(&R.ctl_h->stop, 0u);
   host_store(&R.ctl_h->epoch, ++R.epoch);
   host_store(&R.ctl_h->heartbeat, ++R.heartbeat);
@@ -345,7 +343,7 @@ int resident_create(p2z_server* sv) {
   R->tpw = o.tracks_per_wg ? o.tracks_per_wg : sv->ctx->cfg.tracks_per_wg;
   R->prefetch = o.prefetch ? o.prefetch : 2;
   const int wpc = o.waves_per_cu ? o.waves_per_cu : 8;
-  R->grid = std::max<int64_t>(2, ((int64_t)sv->ctx->cus * wpc * 64 + R->tpw - 1) / R->tpw);
+  R->grid = std::max<int64_t>(2 * p2z::kServeStripes, ((int64_t)sv->ctx->cus * wpc * 64 + R->tpw - 1) / R->tpw);
   R->slot_req.assign(kResSlots, -1);
   for (int32_t i = kResSlots - 1; i >= 0; --i) R->free_slots.push_back(i);
   P2Z_HIP_CHECK(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
