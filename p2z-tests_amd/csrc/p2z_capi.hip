@@ -149,7 +149,7 @@ static int create_impl(p2z_ctx** out, int device, const p2z_opts* opts) {
   if (rc == P2Z_OK) rc = noalloc_scratch_init(device);
   if (rc == P2Z_OK) rc = apply_opts(c, opts);
   if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc((void**)&c->sum_res, 32 * sizeof(double)), "hipMalloc");
-  if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc((void**)&c->counter, 256), "hipMalloc");
+  if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc((void**)&c->counter, p2z::kQueueBytes), "hipMalloc");
   if (rc == P2Z_OK)
     rc = p2z::hip_status(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device),
                          "hipDeviceGetAttribute");

@@ -55,9 +55,15 @@ struct BlockDesc {
 };
 static_assert(sizeof(BlockDesc) == 32, "descriptor layout");
 
+// Work-queue counters of the persistent ragged pass: kQueueStripes counters,
+// kQueueStride 32-bit words (128 bytes) apart, kQueueBytes in all.
+constexpr int kQueueStripes = 8;
+constexpr int kQueueStride = 32;
+constexpr size_t kQueueBytes = (size_t)kQueueStripes * kQueueStride * sizeof(unsigned);
+
 // Ragged pass.  persistent: grid workgroups drain the descriptor table through
-// the device counter (reset first on the same stream), claim items per atomic;
-// otherwise one wave per 64 track slots.
+// the device counters (reset first on the same stream; kQueueBytes), claim
+// items per atomic; otherwise one wave per 64 track slots.
 hipError_t launch_ragged(const float* trk, const float* hit, float* out, const int32_t* lane_nlay,
                          const BlockDesc* blocks, int64_t nblocks, int width, int math, bool persistent,
                          int prefetch, int tpw, int64_t grid, int claim, unsigned* counter, hipStream_t s);

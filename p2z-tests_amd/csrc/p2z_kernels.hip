@@ -180,15 +180,19 @@ __device__ __forceinline__ void ragged_item(const float* __restrict__ trk, const
 // Persistent form: the items are cut into chunks of `claim` consecutive items.
 // Wave w of the grid (nwaves in all) computes chunk w first, with no atomic:
 // a counter that every wave hit at start would serialise thousands of atomics
-// before the first item is computed.  After that a wave takes chunk
-// nwaves + atomicAdd(counter, 1) until it is handed a chunk past the end
-// (every wave reaches that exit).  The atomic is issued when the last item of
-// the current chunk starts, so its round trip overlaps that item; a wave
-// never holds a whole chunk in reserve while others run dry at the end.
-// Every chunk index is handed out once, so every item is computed exactly
-// once.  Several items per claim keep the single counter's atomics
-// (serialised where they meet) few.  The counter starts at 0
-// (k_reset_counter runs first on the same stream).
+// before the first item is computed.  The chunks after those are striped over
+// ns = min(kQueueStripes, workgroups) counters: a wave of workgroup g (stripe
+// s = g % ns) takes chunk nwaves + s + ns * atomicAdd(counter[s], 1) until it
+// is handed a chunk past the end (every wave reaches that exit).  A single
+// counter serialises its atomics where they meet (a few times 1e7 per
+// second); striping spreads them, so that one or two items per claim keep the
+// load even at the end without the counter becoming the limit.  The atomic is
+// issued when the last item of the current chunk starts, so its round trip
+// overlaps that item; a wave never holds a whole chunk in reserve while others
+// run dry at the end.  Each stripe hands out its chunk indexes in order, once
+// each, and every workgroup runs (none waits on another), so every item is
+// computed exactly once.  The counters start at 0 (k_reset_counter runs first
+// on the same stream).
 // FP64 gain: at most 128 VGPRs (4 waves per SIMD instead of 3); with the lane
 // index hidden below, what the limit spills is per-item state, not layer state.
 template <int W, int kMath, int kPrefetch>
@@ -199,13 +203,16 @@ k_ragged_persistent(const float* __restrict__ trk, const float* __restrict__ hit
   const int64_t nitems = (nblocks * W + 63) / 64;
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / 64);
+  const int ns = gridDim.x < (unsigned)kQueueStripes ? (int)gridDim.x : kQueueStripes;
+  const int stripe = (int)(blockIdx.x % (unsigned)ns);
+  unsigned* ctr = counter + stripe * kQueueStride;
   int64_t chunk = (int64_t)blockIdx.x * (blockDim.x / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   while (chunk * claim < nitems) {
     const int64_t first = chunk * claim;
     const int64_t last = first + claim < nitems ? first + claim : nitems;
     unsigned next = 0;
     for (int64_t item = first; item < last; ++item) {
-      if (item == last - 1 && lane == 0) next = atomicAdd(counter, 1u);
+      if (item == last - 1 && lane == 0) next = atomicAdd(ctr, 1u);
       int lane_i = lane;
       // FP64 gain: the lane index hidden from loop-invariant code motion, so
       // that lane-dependent values are not hoisted out of the item loop into
@@ -213,7 +220,7 @@ k_ragged_persistent(const float* __restrict__ trk, const float* __restrict__ hit
       if constexpr (kMath == kFp64Gain) asm volatile("" : "+v"(lane_i));
       ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, lane_i);
     }
-    chunk = nwaves + (int64_t)(unsigned)__shfl((int)next, 0);
+    chunk = nwaves + stripe + (int64_t)ns * (int64_t)(unsigned)__shfl((int)next, 0);
   }
 }
 
@@ -431,7 +438,7 @@ k_serve_resident(ServeLaunch a) {
 }
 
 __global__ void k_reset_counter(unsigned* counter) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < kQueueStripes) counter[threadIdx.x * kQueueStride] = 0u;
 }
 
 // ---------------------------------------------------------------------------

@@ -426,7 +426,7 @@ static int server_create_impl(p2z_ctx* ctx, const void* trk, const void* hit, co
     for (auto& s : sv->slots) {
       if (rc == P2Z_OK) rc = p2z::hip_status(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
       if (rc == P2Z_OK) rc = p2z::hip_status(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate");
-      if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc(&s.counter, 256), "hipMalloc");
+      if (rc == P2Z_OK) rc = p2z::hip_status(hipMalloc(&s.counter, p2z::kQueueBytes), "hipMalloc");
       if (rc == P2Z_OK) rc = slot_reserve(s, kSlotDescriptors);
     }
   }
