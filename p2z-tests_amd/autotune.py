@@ -10,7 +10,7 @@ generator (the tracks do not depend on W, so every candidate does the same
 work).  Results are cached per device and workload kind in
 ~/.cache/p2z/autotune.json unless cache=False.
 
-    python autotune.py [--kind ragged] [--events 200] [--math fp64gain]
+    python autotune.py [--kind ragged] [--events 200] [--math fp64gain] [--seed 17]
     python autotune.py --run FORM,W,TPW,PREFETCH[,WAVES_PER_CU,CLAIM] [--reps N]
         (one configuration, N graph-captured launches: for profiling)
 """
@@ -31,7 +31,7 @@ WIDTHS = (16, 32, 64, 128)
 TPWS = (64, 256)
 PREFETCH = (1, 2)
 WAVES_PER_CU = (8, 16, 24, 32)
-CLAIMS = (2, 4, 8)
+CLAIMS = (1, 2, 4, 8)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "p2z", "autotune.json")
 
 
@@ -139,12 +139,13 @@ def main() -> None:
     ap.add_argument("--math", default="fp64gain", choices=list(p2z.MATH))
     ap.add_argument("--run", default="", help="FORM,W,TPW,PREFETCH[,WAVES_PER_CU,CLAIM]: time one configuration")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=17, help="workload seed of the sweep")
     a = ap.parse_args()
     with p2z.Context(0) as ctx:
         if a.run:
             print(json.dumps(run_one(ctx, a.run, a.kind, a.events, a.math, a.reps)))
             return
-        res = tune(ctx, a.kind, a.events, a.math, log=print)
+        res = tune(ctx, a.kind, a.events, a.math, seed=a.seed, log=print)
     print(json.dumps(res["best"]))
 
 
