@@ -541,7 +541,16 @@ __device__ inline void update_fast(Track& t, const Hit& h, Trig* g = nullptr) {
   const float w = r01 * r01;
   const float e = __builtin_fmaf(-r01, r01, w);
   const float det = __builtin_fmaf(r00, r11, -w) + e;
-  const float s = fast_rcp(det);
+  float s;
+  if (__builtin_expect(!(__builtin_fabsf(det) < 0x1p126f), 0)) {
+    // The FP32 products overflowed (an entry beyond ~1.8e19: a track whose
+    // theta is within rounding of pi/2 turns by 1e4 rad and its covariance
+    // follows), or 1/det would be subnormal: the determinant and its
+    // reciprocal in double, as the reference evaluates them (OMP:519-521).
+    s = (float)(1.0 / ((double)r00 * (double)r11 - (double)r01 * (double)r01));
+  } else {
+    s = fast_rcp(det);
+  }
   const float i00 = s * r11, i01 = r01 * -s, i11 = s * r00;
   float K0[6], K1[6];
 #pragma unroll
