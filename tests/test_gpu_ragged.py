@@ -233,6 +233,23 @@ def test_accuracy_against_fp64(ctx, kind):
     assert g["cov_median"] <= base["cov_median"]
 
 
+def test_ref_mode_full_size_against_oracle(ctx):
+    """The 2000-event ragged workload at full size (6.3e6 tracks, 1.1e8
+    track-layers) in reference typing, through the persistent graph-captured
+    launch: every word equals the oracle's ragged driver (the reference
+    algorithm with glibc's sin/cos, on all host cores)."""
+    W = 32
+    w = p2z.Workload("ragged", nevents=2000, width=W, seed=2024)
+    log(f"case ragged-ref/full-size tracks={w.ntracks} track-layers={w.track_layers}")
+    d = Dev(ctx, w)
+    ctx.run_ragged(d.trk, d.hit, d.out, d.lanes, d.blk, w.nblocks, W, "ref", graph=True)
+    out = ctx.download(d.out)
+    d.free()
+    exp = oracle_ragged(w)
+    log(f"  words equal {words_equal(out, exp):.8f}")
+    assert same_words(out, exp)
+
+
 def test_full_size_forms_agree(ctx):
     """The 2000-event ragged workload at full size (about 1.1e8 track-layers):
     the persistent launch (default shape and a grid smaller than the work),
