@@ -30,7 +30,7 @@ import p2z  # noqa: E402
 WIDTHS = (16, 32, 64, 128)
 TPWS = (64, 256)
 PREFETCH = (1, 2)
-WAVES_PER_CU = (8, 16, 24, 32)
+WAVES_PER_CU = (8, 12, 16, 20, 32)
 CLAIMS = (1, 2, 4, 8)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "p2z", "autotune.json")
 

@@ -472,7 +472,12 @@ static int run_ragged_impl(p2z_ctx* c, const void* trk, const void* hit, void* o
   P2Z_TRY(use_device(c));
   if (nblocks == 0) return P2Z_OK;
   const bool persistent = flags & P2Z_RAGGED_PERSISTENT;
-  const int64_t grid = ((int64_t)c->cus * wpc * 64 + tpw - 1) / tpw;
+  int64_t grid = ((int64_t)c->cus * wpc * 64 + tpw - 1) / tpw;
+  if (persistent) {
+    // no more workgroups than the CUs hold at once
+    const int per_cu = p2z::ragged_persistent_groups_per_cu(width, math, prefetch, tpw);
+    if (per_cu > 0) grid = std::min<int64_t>(grid, (int64_t)per_cu * c->cus);
+  }
   const auto* bd = reinterpret_cast<const p2z::BlockDesc*>(blocks);
   auto launch = [&](hipStream_t s) {
     return p2z::launch_ragged((const float*)trk, (const float*)hit, (float*)out, lane_nlayer, bd, nblocks, width,

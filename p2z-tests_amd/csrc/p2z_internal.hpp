@@ -68,6 +68,12 @@ hipError_t launch_ragged(const float* trk, const float* hit, float* out, const i
                          const BlockDesc* blocks, int64_t nblocks, int width, int math, bool persistent,
                          int prefetch, int tpw, int64_t grid, int claim, unsigned* counter, hipStream_t s);
 constexpr int kDefaultClaim = 4;
+// Workgroups of tpw threads of the persistent ragged kernel (width, math,
+// prefetch) that one CU holds at once (0 if the runtime cannot tell).  A
+// persistent grid larger than that many per CU leaves waves that start only
+// when resident ones have ended, and their first chunks, handed out without
+// a claim, are then the longest items computed last.
+int ragged_persistent_groups_per_cu(int width, int math, int prefetch, int tpw);
 
 // Report statistics (OMP:758-851) on device.  part: scratch of
 // summary_scratch_doubles(nevts, ntrks) doubles.  res: 18 doubles on device

@@ -10,6 +10,9 @@
 #include "p2z_lane.hpp"
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 
 namespace p2z {
 namespace {
@@ -193,8 +196,7 @@ __device__ __forceinline__ void ragged_item(const float* __restrict__ trk, const
 // each, and every workgroup runs (none waits on another), so every item is
 // computed exactly once.  The counters start at 0 (k_reset_counter runs first
 // on the same stream).
-// FP64 gain: at most 128 VGPRs (4 waves per SIMD instead of 3); with the lane
-// index hidden below, what the limit spills is per-item state, not layer state.
+// FP64 gain: at most 128 VGPRs (4 waves per SIMD instead of 3).
 template <int W, int kMath, int kPrefetch>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMath == kFp64Gain ? 4 : 1))) void
 k_ragged_persistent(const float* __restrict__ trk, const float* __restrict__ hit, float* __restrict__ out,
@@ -214,13 +216,16 @@ k_ragged_persistent(const float* __restrict__ trk, const float* __restrict__ hit
     for (int64_t item = first; item < last; ++item) {
       if (item == last - 1 && lane == 0) next = atomicAdd(ctr, 1u);
       int lane_i = lane;
-      // FP64 gain: the lane index hidden from loop-invariant code motion, so
-      // that lane-dependent values are not hoisted out of the item loop into
-      // registers held across all of it (10 VGPRs)
-      if constexpr (kMath == kFp64Gain) asm volatile("" : "+v"(lane_i));
+      // the lane index hidden from loop-invariant code motion, so that
+      // lane-dependent values are not hoisted out of the item loop into
+      // registers held across all of it (10 VGPRs in every mode: FAST then
+      // fits 96 VGPRs, 5 waves per SIMD, as the one-shot form does)
+      asm volatile("" : "+v"(lane_i));
       ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, lane_i);
     }
-    chunk = nwaves + stripe + (int64_t)ns * (int64_t)(unsigned)__shfl((int)next, 0);
+    // lane 0's claim read into a scalar: the chunk index, and with it the
+    // loop's control, stay wave-uniform scalars
+    chunk = nwaves + stripe + (int64_t)ns * (int64_t)__builtin_amdgcn_readlane(next, 0);
   }
 }
 
@@ -414,7 +419,9 @@ __device__ void serve_work(const ServeLaunch& a, int lane) {
     if (have) {
       uint64_t next = 0;
       if (lane == 0) next = __hip_atomic_fetch_add(head, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      serve_item<W, kMath, kPrefetch>(a, k, lane);
+      int lane_i = lane;
+      asm volatile("" : "+v"(lane_i));  // as in the persistent ragged pass
+      serve_item<W, kMath, kPrefetch>(a, k, lane_i);
       k = lane0_u64(next) * kServeStripes + stripe;
     }
   }
@@ -768,6 +775,30 @@ hipError_t launch_update_v1(float* trkErr, float* inPar, const float* hitErr, co
                          inPar, hitErr, msP, ntracks, ts, hs);
     return hipGetLastError();
   });
+}
+
+int ragged_persistent_groups_per_cu(int width, int math, int prefetch, int tpw) {
+  static std::mutex mu;
+  static std::map<std::array<int, 4>, int> cache;
+  const std::array<int, 4> key{width, math, prefetch >= 2 ? 2 : 1, tpw};
+  std::lock_guard<std::mutex> lock(mu);
+  const auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int n = 0;
+  const hipError_t e = dispatch_width(width, [&](auto wc) {
+    return dispatch_math(math, [&](auto mc) {
+      constexpr int W = decltype(wc)::value, M = decltype(mc)::value;
+      const void* k = prefetch >= 2 ? reinterpret_cast<const void*>(&k_ragged_persistent<W, M, 2>)
+                                    : reinterpret_cast<const void*>(&k_ragged_persistent<W, M, 1>);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, tpw, 0);
+    });
+  });
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  cache.emplace(key, n);
+  return n;
 }
 
 hipError_t launch_ragged(const float* trk, const float* hit, float* out, const int32_t* lane_nlay,
