@@ -64,7 +64,7 @@ class Arena {
 };
 
 struct Request {
-  std::vector<p2z_block_desc> blocks;  // out_block rewritten to arena positions
+  std::vector<p2z_block_desc> blocks;  // batches form: out_block rewritten to arena positions
   int64_t out_block = 0, nblocks = 0;
   double t_arrival = 0, t_admit = 0, t_launch = 0, t_done = 0;
   bool done = false;
@@ -88,6 +88,7 @@ struct Slot {
 constexpr int64_t kResItems = 65536;  // work-item ring (9 MiB of pinned memory)
 constexpr int32_t kResSlots = 4096;   // requests in flight; also the completion ring
 constexpr double kResIdleSeconds = 2.0;
+constexpr double kBeatSeconds = 0.05;
 
 struct Resident {
   hipStream_t stream = nullptr;
@@ -101,9 +102,11 @@ struct Resident {
   uint64_t published = 0;             // items published (ServeCtl::tail)
   uint64_t comp_read = 0;             // completion records consumed
   uint64_t heartbeat = 0;
+  double t_beat = 0;                  // host time of the last heartbeat store
   std::vector<int64_t> slot_req;      // request of each slot (-1: free)
   std::vector<int32_t> free_slots;
   std::map<uint64_t, int64_t> unfinished;  // first item -> request, in publication order
+  std::vector<p2z_block_desc> sorted;      // a request's blocks put longest first
   bool running = false;
   uint32_t epoch = 0;  // launches so far (ServeCtl::epoch of the current one)
   int64_t launches = 0;
@@ -199,11 +202,14 @@ int launch_batches(p2z_server* sv) {
                        [](const p2z_block_desc& a, const p2z_block_desc& b) { return a.nlayer > b.nlayer; });
       const int tpw = o.tracks_per_wg ? o.tracks_per_wg : sv->ctx->cfg.tracks_per_wg;
       const int wpc = o.waves_per_cu ? o.waves_per_cu : 8;
-      const int64_t grid = ((int64_t)sv->ctx->cus * wpc * 64 + tpw - 1) / tpw;
+      const int pf = o.prefetch ? o.prefetch : 2;
+      int64_t grid = ((int64_t)sv->ctx->cus * wpc * 64 + tpw - 1) / tpw;
+      const int per_cu = p2z::ragged_persistent_groups_per_cu(o.width, o.math, pf, tpw);
+      if (per_cu > 0) grid = std::min<int64_t>(grid, (int64_t)per_cu * sv->ctx->cus);
       P2Z_HIP_CHECK(
           hipMemcpyAsync(s.d_desc, s.h_desc, nblk * sizeof(p2z_block_desc), hipMemcpyHostToDevice, s.stream));
       P2Z_HIP_CHECK(p2z::launch_ragged(sv->trk, sv->hit, sv->arena_mem, sv->lanes, s.d_desc, nblk, o.width, o.math,
-                                       true, o.prefetch ? o.prefetch : 2, tpw, grid,
+                                       true, pf, tpw, grid,
                                        o.items_per_claim ? o.items_per_claim : 1, s.counter, s.stream));
       P2Z_HIP_CHECK(hipEventRecord(s.done, s.stream));
       return P2Z_OK;
@@ -245,6 +251,7 @@ int resident_launch(p2z_server* sv) {
   host_store(&R.ctl_h->stop, 0u);
   host_store(&R.ctl_h->epoch, ++R.epoch);
   host_store(&R.ctl_h->heartbeat, ++R.heartbeat);
+  R.t_beat = now_s();
   p2z::ServeLaunch a{};
   a.trk = sv->trk;
   a.hit = sv->hit;
@@ -270,7 +277,13 @@ int resident_launch(p2z_server* sv) {
 // item below its final tail.
 int resident_keepalive(p2z_server* sv) {
   Resident& R = *sv->res;
-  host_store(&R.ctl_h->heartbeat, ++R.heartbeat);
+  // the dispatcher polls this word over PCIe: one store per kBeatSeconds is
+  // enough against an idle timeout of kResIdleSeconds
+  const double now = now_s();
+  if (now - R.t_beat >= kBeatSeconds) {
+    host_store(&R.ctl_h->heartbeat, ++R.heartbeat);
+    R.t_beat = now;
+  }
   if (R.running && host_load(&R.ctl_h->exited) == R.epoch) R.running = false;
   if (!R.running && !R.unfinished.empty()) P2Z_TRY(resident_launch(sv));
   return P2Z_OK;
@@ -306,21 +319,28 @@ int64_t resident_items(int64_t nblocks, int W) {
 
 // Publishes an admitted request: its descriptors longest-first, packed into
 // work items in the ring, then one release store of the new tail.
-void resident_publish(p2z_server* sv, int64_t id, Request& r, int32_t slot) {
+void resident_publish(p2z_server* sv, int64_t id, const p2z_block_desc* blocks, Request& r, int32_t slot) {
   Resident& R = *sv->res;
   const int W = sv->opts.width;
-  std::vector<p2z_block_desc> d = r.blocks;
-  std::stable_sort(d.begin(), d.end(),
-                   [](const p2z_block_desc& a, const p2z_block_desc& b) { return a.nlayer > b.nlayer; });
-  const int64_t n = resident_items(r.nblocks, W);
+  const int64_t nb = r.nblocks;
+  // longest first; an event's blocks usually arrive that way already
+  const p2z_block_desc* d = blocks;
+  auto longer = [](const p2z_block_desc& a, const p2z_block_desc& b) { return a.nlayer > b.nlayer; };
+  if (!std::is_sorted(blocks, blocks + nb, longer)) {
+    R.sorted.assign(blocks, blocks + nb);
+    std::stable_sort(R.sorted.begin(), R.sorted.end(), longer);
+    d = R.sorted.data();
+  }
+  const int64_t n = resident_items(nb, W);
   const int per = W <= 64 ? 64 / W : 1;
+  const int64_t at = r.out_block;
   for (int64_t i = 0; i < n; ++i) {
     p2z::ServeItem& it = R.items_h[(R.published + i) % kResItems];
     const int64_t first = W <= 64 ? i * per : i / 2;
-    const int nd = (int)std::min<int64_t>(per, r.nblocks - first);
+    const int nd = (int)std::min<int64_t>(per, nb - first);
     for (int k = 0; k < nd; ++k) {
       const p2z_block_desc& s = d[first + k];
-      it.desc[k] = p2z::BlockDesc{s.trk_block, s.hit_block, s.out_block, s.nlayer, s.event};
+      it.desc[k] = p2z::BlockDesc{s.trk_block, s.hit_block, s.out_block + at, s.nlayer, s.event};
     }
     it.ndesc = nd;
     it.half = W <= 64 ? 0 : (int32_t)(i % 2);
@@ -487,20 +507,21 @@ static int server_submit_impl(p2z_server* sv, int64_t request, const p2z_block_d
   if (at < 0) return p2z::fail(P2Z_EBUSY, "arena full (%lld of %lld blocks in use)", (long long)sv->arena->used(),
                                (long long)sv->arena->total());
   Request r;
-  r.blocks.assign(blocks, blocks + nblocks);
-  for (auto& d : r.blocks) d.out_block += at;
   r.out_block = at;
   r.nblocks = nblocks;
   r.t_arrival = t_arrival;
   r.t_admit = now_s();
   if (sv->res) {
+    // the loop reads its descriptors from the ring: the request keeps none
     Resident& R = *sv->res;
     const int32_t slot = R.free_slots.back();
     R.free_slots.pop_back();
     Request& rr = sv->reqs.emplace(request, std::move(r)).first->second;
-    resident_publish(sv, request, rr, slot);
+    resident_publish(sv, request, blocks, rr, slot);
     return resident_keepalive(sv);
   }
+  r.blocks.assign(blocks, blocks + nblocks);
+  for (auto& d : r.blocks) d.out_block += at;
   sv->reqs.emplace(request, std::move(r));
   sv->pending.push_back(request);
   return P2Z_OK;
