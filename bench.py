@@ -240,7 +240,8 @@ def section_ragged(ctx, nevents: int, sample_events: int) -> tuple[dict, dict]:
         log(f"ragged: {math} best {best}")
         row = {"tuned": {"form": best["form"], "persistent": dict(width=bp["width"], **opts),
                          "oneshot": dict(width=bo["width"], **oopts),
-                         "candidates": tuned["candidates"], "sample_events": sample_events,
+                         "candidates": tuned["candidates"], "refined": tuned["refined"],
+                         "sample_events": sample_events,
                          "tuning_set": f"seed {TUNE_SEED} (held out; evaluation set seed {EVAL_SEED})",
                          "tuning_s": round(t_tune, 2)}}
 
@@ -248,20 +249,26 @@ def section_ragged(ctx, nevents: int, sample_events: int) -> tuple[dict, dict]:
             return {"ms": round(ms, 4), "tl_per_s": w.track_layers / (ms * 1e-3),
                     "hbm_frac": round(ragged_bytes(w) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
 
-        W = bp["width"]
-        w, d = pool(W)
-        res["workload"] = (f"{nevents} events, {w.ntracks} tracks, {w.track_layers} track-layers "
+        # the two forms timed in alternation, three rounds each (median):
+        # the clock history of a long run shifts a single round by a few %
+        Wp, Wo = bp["width"], bo["width"]
+        wp, dp = pool(Wp)
+        wo, do = pool(Wo)
+        res["workload"] = (f"{nevents} events, {wp.ntracks} tracks, {wp.track_layers} track-layers "
                            f"(tracks/event log-normal median 2000 sigma 1 in [50,20000], layers U{{8..26}})")
-        out = ctx.alloc(w.tracks.nbytes)
-        row["persistent_graph"] = dict(width=W, **rate(w, time_ms(ctx, lambda: ctx.run_ragged(
-            d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, math, graph=True, **opts), 20)))
-        out.free()
-        W = bo["width"]
-        w, d = pool(W)
-        out = ctx.alloc(w.tracks.nbytes)
-        row["oneshot_graph"] = dict(width=W, **rate(w, time_ms(ctx, lambda: ctx.run_ragged(
-            d["trk"], d["hit"], out, d["lanes"], d["lpt"], w.nblocks, W, math, persistent=False, graph=True,
-            **oopts), 20)))
+        outp, outo = ctx.alloc(wp.tracks.nbytes), ctx.alloc(wo.tracks.nbytes)
+        tp, to = [], []
+        for _ in range(3):
+            tp.append(time_ms(ctx, lambda: ctx.run_ragged(dp["trk"], dp["hit"], outp, dp["lanes"], dp["lpt"],
+                                                          wp.nblocks, Wp, math, graph=True, **opts), 20))
+            to.append(time_ms(ctx, lambda: ctx.run_ragged(do["trk"], do["hit"], outo, do["lanes"], do["lpt"],
+                                                          wo.nblocks, Wo, math, persistent=False, graph=True,
+                                                          **oopts), 20))
+        row["persistent_graph"] = dict(width=Wp, **rate(wp, float(np.median(tp))),
+                                       rounds_ms=[round(t, 4) for t in tp])
+        row["oneshot_graph"] = dict(width=Wo, **rate(wo, float(np.median(to))), rounds_ms=[round(t, 4) for t in to])
+        outp.free()
+        W, w, d, out = Wo, wo, do, outo
         eb = w.event_blocks
 
         def per_event():

@@ -4,8 +4,10 @@ Sweeps the launch form (persistent work queue or one-shot), the AoSoA width
 W in {16, 32, 64, 128}, tracks per workgroup, hit prefetch depth and, for the
 persistent form, the grid size (waves per CU) and the items per claim on a
 sample of the workload, timing each candidate with hipEvents (sustained time
-per launch over back-to-back launches, median of three rounds), and returns
-the fastest.  The sample is generated at each width by the same
+per launch over back-to-back launches, median of three rounds).  The closest
+candidates of each form are then timed again with rounds four times as long
+(short rounds rank near-ties by noise), and the fastest of those is
+returned.  The sample is generated at each width by the same
 generator (the tracks do not depend on W, so every candidate does the same
 work).  Results are cached per device and workload kind in
 ~/.cache/p2z/autotune.json unless cache=False.
@@ -56,42 +58,58 @@ def _time(ctx, fn, reps: int, rounds: int = 3) -> float:
 
 def tune(ctx, kind: str = "ragged", nevents: int = 200, math: str = "fp64gain", seed: int = 17,
          reps: int = 5, widths=WIDTHS, tpws=TPWS, prefetch=PREFETCH, waves=WAVES_PER_CU, claims=CLAIMS,
-         forms=("persistent", "oneshot"), log=lambda s: None) -> dict:
+         forms=("persistent", "oneshot"), refine: int = 6, log=lambda s: None) -> dict:
     """Times every candidate launch on the sample and returns the fastest
     overall (best) and the fastest of each launch form (best_<form>).  The
     persistent form sweeps tracks per workgroup x prefetch depth x waves per
     CU x items per claim; the one-shot form (one wave per 64 track slots, dispatched by the
-    hardware) sweeps tracks per workgroup x prefetch depth."""
-    results = []
-    for W in widths:
+    hardware) sweeps tracks per workgroup x prefetch depth.  The `refine`
+    fastest candidates of each form are timed again with 4 * reps launches
+    per round, and the best_* entries come from that second timing."""
+
+    def timed(W, cands, r):
         w = p2z.Workload(kind, nevents=nevents, width=W, seed=seed)
         trk, hit = ctx.upload(w.tracks), ctx.upload(w.hits)
         lanes = ctx.upload(np.ascontiguousarray(w.lane_nlayer))
         blk = ctx.upload(np.ascontiguousarray(w.blocks_lpt))
         out = ctx.alloc(w.tracks.nbytes)
+        recs = []
+        for form, tpw, pf, wpc, cl in cands:
+            pers = form == "persistent"
+            ms = _time(ctx, lambda: ctx.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, math, persistent=pers,
+                                                   tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc,
+                                                   items_per_claim=cl), r)
+            rec = dict(form=form, width=W, tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc, items_per_claim=cl,
+                       ms=ms, tl_per_s=w.track_layers / (ms * 1e-3))
+            recs.append(rec)
+            log(f"  {form:10s} W={W:3d} tpw={tpw:3d} prefetch={pf} waves/CU={wpc:2d} claim={cl}: {ms:.4f} ms  "
+                f"{rec['tl_per_s']:.3e} TL/s" + ("  (refined)" if r != reps else ""))
+        for b in (trk, hit, lanes, blk, out):
+            b.free()
+        w.close()
+        return recs
+
+    results = []
+    for W in widths:
         cands = []
         if "persistent" in forms:
             cands += [("persistent", tpw, pf, wpc, cl)
                       for tpw, pf, wpc, cl in itertools.product(tpws, prefetch, waves, claims)]
         if "oneshot" in forms:
             cands += [("oneshot", tpw, pf, 0, 0) for tpw, pf in itertools.product(tpws, prefetch)]
-        for form, tpw, pf, wpc, cl in cands:
-            pers = form == "persistent"
-            ms = _time(ctx, lambda: ctx.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, math, persistent=pers,
-                                                   tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc,
-                                                   items_per_claim=cl), reps)
-            rec = dict(form=form, width=W, tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc, items_per_claim=cl,
-                       ms=ms, tl_per_s=w.track_layers / (ms * 1e-3))
-            results.append(rec)
-            log(f"  {form:10s} W={W:3d} tpw={tpw:3d} prefetch={pf} waves/CU={wpc:2d} claim={cl}: {ms:.4f} ms  "
-                f"{rec['tl_per_s']:.3e} TL/s")
-        for b in (trk, hit, lanes, blk, out):
-            b.free()
-        w.close()
-    res = dict(best=min(results, key=lambda r: r["ms"]), kind=kind, math=math, sample_events=nevents,
-               candidates=len(results), results=results)
+        results += timed(W, cands, reps)
+    key = lambda r: (r["form"], r["tracks_per_wg"], r["prefetch"], r["waves_per_cu"], r["items_per_claim"])
+    refined = []
+    if refine > 0:
+        top = [r for form in forms
+               for r in sorted((r for r in results if r["form"] == form), key=lambda r: r["ms"])[:refine]]
+        for W in sorted({r["width"] for r in top}):
+            refined += timed(W, [key(r) for r in top if r["width"] == W], 4 * reps)
+    ranked = refined or results
+    res = dict(best=min(ranked, key=lambda r: r["ms"]), kind=kind, math=math, sample_events=nevents,
+               candidates=len(results), refined=len(refined), results=results, refined_results=refined)
     for form in forms:
-        rs = [r for r in results if r["form"] == form]
+        rs = [r for r in ranked if r["form"] == form]
         if rs:
             res["best_" + form] = min(rs, key=lambda r: r["ms"])
     return res

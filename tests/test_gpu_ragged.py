@@ -3,6 +3,9 @@ launch, hipGraph replay, one-shot launch, all widths, against the oracle's
 ragged driver and against the plain fused path."""
 from __future__ import annotations
 
+import os
+import sys
+
 import numpy as np
 import pytest
 
@@ -288,3 +291,22 @@ def test_full_size_forms_agree(ctx):
     assert np.array_equal(t[bi, 2, li], zlast)
     assert np.all(np.isfinite(t[bi, :6, li]))
     d.free()
+
+
+def test_autotuner_refines_its_closest_candidates(ctx):
+    """The tuner's sweep and its second, longer timing of the closest
+    candidates of each form, on a small sample: every returned best is one of
+    the re-timed candidates, and each form's best is the fastest of them."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "p2z-tests_amd"))
+    import autotune
+
+    res = autotune.tune(ctx, "ragged", nevents=24, math="fast", seed=3, reps=2, widths=(32, 64), tpws=(64,),
+                        prefetch=(1,), waves=(8, 16), claims=(1, 2), refine=2)
+    log(f"case autotune small sample: {res['candidates']} candidates, {res['refined']} refined")
+    assert res["candidates"] == 2 * (4 + 1)
+    assert res["refined"] == 2 + 2
+    for form in ("persistent", "oneshot"):
+        rs = [r for r in res["refined_results"] if r["form"] == form]
+        assert len(rs) == 2
+        assert res["best_" + form] == min(rs, key=lambda r: r["ms"])
+    assert res["best"] in res["refined_results"]
