@@ -338,21 +338,31 @@ template <int W, int kMath, int kPrefetch>
 __device__ __forceinline__ void serve_item(const ServeLaunch& a, uint64_t k, int lane) {
   const ServeItem* it = a.items + (int64_t)(k % (uint64_t)a.item_cap);
   const int di = W <= 64 ? lane / W : 0;
-  // the item's header and this lane's descriptor in one round trip (a
-  // padding descriptor is read too: it lies inside the record)
-  const int ndesc_v = sys_load(&it->ndesc);
-  const int half_v = sys_load(&it->half);
-  const int req_slot_v = sys_load(&it->req_slot);
-  const int req_items_v = sys_load(&it->req_items);
-  const BlockDesc* d = &it->desc[di];
-  const int64_t tb = sys_load(&d->trk_block);
-  const int64_t hblk = sys_load(&d->hit_block);
-  const int64_t ob = sys_load(&d->out_block);
-  const int nl = sys_load(&d->nlayer);
-  const int ndesc = (int)lane0_u32((uint32_t)ndesc_v);
-  const int req_slot = (int)lane0_u32((uint32_t)req_slot_v);
-  const uint32_t req_items = lane0_u32((uint32_t)req_items_v);
-  const int l = W <= 64 ? lane % W : (int)lane0_u32((uint32_t)half_v) * 64 + lane;
+  // The whole 144-byte record in one load instruction, one round trip over
+  // PCIe: lane j < 18 reads the record's 8-byte word j (18 lanes of
+  // consecutive words: a few PCIe requests, where a load per field costs one
+  // request each).  Each lane then takes its descriptor's fields from the
+  // lanes that hold them: desc[d] is words 4d .. 4d + 3 (trk_block,
+  // hit_block, out_block, nlayer | event), the header words 16 (ndesc | half)
+  // and 17 (req_slot | req_items).  A padding descriptor is read too: it lies
+  // inside the record.
+  static_assert(sizeof(ServeItem) == 18 * sizeof(uint64_t), "serve item: 18 words");
+  static_assert(offsetof(ServeItem, ndesc) == 16 * 8 && offsetof(ServeItem, req_slot) == 17 * 8, "header words");
+  uint64_t word = 0;
+  if (lane < 18) word = sys_load(reinterpret_cast<const uint64_t*>(it) + lane);
+  const int wlo = (int)(uint32_t)word, whi = (int)(uint32_t)(word >> 32);
+  auto pick = [&](int j) -> int64_t {
+    return (int64_t)(((uint64_t)(uint32_t)__shfl(whi, j) << 32) | (uint32_t)__shfl(wlo, j));
+  };
+  const int64_t tb = pick(4 * di);
+  const int64_t hblk = pick(4 * di + 1);
+  const int64_t ob = pick(4 * di + 2);
+  const int nl = __shfl(wlo, 4 * di + 3);
+  const int ndesc = (int)__builtin_amdgcn_readlane((uint32_t)wlo, 16);
+  const int half = (int)__builtin_amdgcn_readlane((uint32_t)whi, 16);
+  const int req_slot = (int)__builtin_amdgcn_readlane((uint32_t)wlo, 17);
+  const uint32_t req_items = __builtin_amdgcn_readlane((uint32_t)whi, 17);
+  const int l = W <= 64 ? lane % W : half * 64 + lane;
   const bool valid = di < ndesc;
   Track t{};
   int n = 0;
