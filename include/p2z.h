@@ -293,7 +293,9 @@ int p2z_workload_load(const char* path, p2z_workload** out);
 typedef struct {
   int64_t budget_bytes;     /* output arena size                               */
   int32_t nstreams;         /* batches in flight at once, 1..16; 0 = 4         */
-  int32_t max_batch_blocks; /* block limit of one batch; 0 = none              */
+  int32_t max_batch_blocks; /* block limit of one batch (a request above it
+                               runs as a batch of its own); 0 = 65536, the
+                               descriptor room reserved per stream at creation */
   int32_t width, math;      /* AoSoA width of the pool, arithmetic mode        */
   int32_t tracks_per_wg, prefetch, waves_per_cu, items_per_claim; /* as p2z_ragged_opts, except
                                items_per_claim 0 = 1: a batch of one event

@@ -133,6 +133,7 @@ struct p2z_server {
   std::vector<Slot> slots;
   std::vector<p2z_completion> ready;
   int64_t batches = 0, batch_events = 0;
+  std::vector<int64_t> sort_start;  // counting-sort offsets of launch_batches
   Resident* res = nullptr;  // non-null: the device-resident loop serves
 };
 
@@ -178,6 +179,39 @@ int harvest(p2z_server* sv) {
   return P2Z_OK;
 }
 
+// The blocks of the requests `ids`, longest first (stable: requests in order,
+// each request's blocks in its order), written to dst.  Layer counts span a
+// few dozen values, so a counting sort over them does it in two passes; a
+// comparison sort of a large batch (tens of thousands of blocks) took
+// milliseconds of host time while the device waited for the batch.
+void gather_longest_first(p2z_server* sv, const std::vector<int64_t>& ids, p2z_block_desc* dst, int64_t nblk) {
+  constexpr int32_t kMaxCounted = 4096;
+  int32_t top = 0;
+  for (int64_t id : ids)
+    for (const auto& d : sv->reqs.at(id).blocks) top = std::max(top, d.nlayer);
+  if (top >= kMaxCounted) {
+    int64_t k = 0;
+    for (int64_t id : ids)
+      for (const auto& d : sv->reqs.at(id).blocks) dst[k++] = d;
+    std::stable_sort(dst, dst + nblk,
+                     [](const p2z_block_desc& a, const p2z_block_desc& b) { return a.nlayer > b.nlayer; });
+    return;
+  }
+  // start[c]: first position of the blocks with c layers (descending c)
+  std::vector<int64_t>& start = sv->sort_start;
+  start.assign((size_t)top + 2, 0);
+  for (int64_t id : ids)
+    for (const auto& d : sv->reqs.at(id).blocks) ++start[(size_t)d.nlayer];
+  int64_t at = 0;
+  for (int32_t c = top; c >= 0; --c) {
+    const int64_t n = start[(size_t)c];
+    start[(size_t)c] = at;
+    at += n;
+  }
+  for (int64_t id : ids)
+    for (const auto& d : sv->reqs.at(id).blocks) dst[start[(size_t)d.nlayer]++] = d;
+}
+
 int launch_batches(p2z_server* sv) {
   const p2z_server_opts& o = sv->opts;
   for (auto& s : sv->slots) {
@@ -195,11 +229,7 @@ int launch_batches(p2z_server* sv) {
     }
     const int rc = [&]() -> int {
       P2Z_TRY(slot_reserve(s, nblk));
-      int64_t k = 0;
-      for (int64_t id : ids)
-        for (const auto& d : sv->reqs.at(id).blocks) s.h_desc[k++] = d;
-      std::stable_sort(s.h_desc, s.h_desc + nblk,
-                       [](const p2z_block_desc& a, const p2z_block_desc& b) { return a.nlayer > b.nlayer; });
+      gather_longest_first(sv, ids, s.h_desc, nblk);
       const int tpw = o.tracks_per_wg ? o.tracks_per_wg : sv->ctx->cfg.tracks_per_wg;
       const int wpc = o.waves_per_cu ? o.waves_per_cu : 8;
       const int pf = o.prefetch ? o.prefetch : 2;
@@ -415,6 +445,11 @@ static int server_create_impl(p2z_ctx* ctx, const void* trk, const void* hit, co
   if (o.nstreams == 0) o.nstreams = 4;
   if (o.nstreams < 1 || o.nstreams > 16) return p2z::fail(P2Z_EINVAL, "nstreams must be in [1,16]");
   if (o.max_batch_blocks < 0) return p2z::fail(P2Z_EINVAL, "max_batch_blocks must be >= 0");
+  // by default a batch fits the descriptor room each stream reserves here: a
+  // larger one would reallocate it inside the loop, and freeing device memory
+  // waits for every batch in flight (a request larger than the limit still
+  // runs, as a batch of its own)
+  if (o.max_batch_blocks == 0) o.max_batch_blocks = (int32_t)kSlotDescriptors;
   if (o.tracks_per_wg && (o.tracks_per_wg < 64 || o.tracks_per_wg > 256 || o.tracks_per_wg % 64))
     return p2z::fail(P2Z_EINVAL, "tracks_per_wg must be 64, 128, 192 or 256");
   if (o.prefetch < 0 || o.prefetch > 2) return p2z::fail(P2Z_EINVAL, "prefetch must be 0, 1 or 2");
