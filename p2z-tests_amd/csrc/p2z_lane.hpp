@@ -281,11 +281,22 @@ __device__ inline void sincos_unit(float x, float* sp, float* cp, float* omcp) {
 // sincos_unit where |x| <= 1 (the common case for the turning angle of one
 // layer), the library's reduced-argument sincosf elsewhere.  NaN takes the
 // library branch.
+// The library's sincosf, kept out of line: inline, its reduction of large
+// arguments (64-bit products) would add its registers to the register peak of
+// every layer loop that calls it, although track angles never need it.
+__device__ __noinline__ float2 sincosf_outline(float x) {
+  float s, c;
+  sincosf(x, &s, &c);
+  return make_float2(s, c);
+}
+
 __device__ inline void sincos_short(float x, float* sp, float* cp, float* omcp) {
   if (__builtin_fabsf(x) <= 1.0f) {
     sincos_unit(x, sp, cp, omcp);
   } else {
-    sincosf(x, sp, cp);
+    const float2 v = sincosf_outline(x);
+    *sp = v.x;
+    *cp = v.y;
     *omcp = 1.0f - *cp;
   }
 }
@@ -333,8 +344,11 @@ __device__ inline void trig_init(const Track& t, Trig& g) {
     sincos_accurate(t.p[4], &g.sinP, &g.cosP);
     sincos_accurate(t.p[5], &g.sinT, &g.cosT);
   } else {
-    sincosf(t.p[4], &g.sinP, &g.cosP);
-    sincosf(t.p[5], &g.sinT, &g.cosT);
+    const float2 a = sincosf_outline(t.p[4]), b = sincosf_outline(t.p[5]);
+    g.sinP = a.x;
+    g.cosP = a.y;
+    g.sinT = b.x;
+    g.cosT = b.y;
   }
 }
 
@@ -524,6 +538,15 @@ struct Live {
 
 // kCarry: g holds sin/cos of phi and theta, rotated here by the update's
 // corrections of the two angles.
+// 1 / (r00 r11 - r01^2) with the determinant and the division in double,
+// rounded to float.  Out of line: the update takes it only for operands whose
+// FP32 determinant overflows, and inline its double division's registers
+// would add to the register peak of every layer loop.
+__device__ __noinline__ float inv_det_double(float r00, float r01, float r11) {
+#pragma clang fp contract(off)
+  return (float)(1.0 / ((double)r00 * (double)r11 - (double)r01 * (double)r01));
+}
+
 template <bool kZRow0, bool kCarry = false>
 __device__ inline void update_fast(Track& t, const Hit& h, Trig* g = nullptr) {
 #pragma clang fp contract(fast)
@@ -547,7 +570,7 @@ __device__ inline void update_fast(Track& t, const Hit& h, Trig* g = nullptr) {
     // theta is within rounding of pi/2 turns by 1e4 rad and its covariance
     // follows), or 1/det would be subnormal: the determinant and its
     // reciprocal in double, as the reference evaluates them (OMP:519-521).
-    s = (float)(1.0 / ((double)r00 * (double)r11 - (double)r01 * (double)r01));
+    s = inv_det_double(r00, r01, r11);
   } else {
     s = fast_rcp(det);
   }
@@ -609,18 +632,20 @@ __device__ inline void update_fp64gain(Track& t, const Hit& h, Trig* g = nullptr
   constexpr int n = L::n;
   float* p = t.p;
   float* P = t.P;
-  // the live block in double, packed over live indices
-  double Q[21];
+  // entry (a, b) of the live block in double (a and b over live indices)
+  auto q = [&](int a, int b) -> double { return (double)P[S(L::at(a), L::at(b))]; };
+  // C = P H^T: the columns of x and y
+  double C0[6], C1[6];
 #pragma unroll
-  for (int a = 0; a < n; ++a)
-#pragma unroll
-    for (int b = 0; b <= a; ++b) Q[a * (a + 1) / 2 + b] = (double)P[S(L::at(a), L::at(b))];
-  auto q = [&](int a, int b) -> double { return a >= b ? Q[a * (a + 1) / 2 + b] : Q[b * (b + 1) / 2 + a]; };
+  for (int a = 0; a < n; ++a) {
+    C0[a] = q(a, 0);
+    C1[a] = q(a, 1);
+  }
   const double R00 = h.xx, R01 = h.xy, R11 = h.yy;
   // S = H P H^T + R and its inverse: 1/det from the hardware FP64 reciprocal
   // refined by two Newton steps (each doubles the correct bits; the
   // estimate has more than 20), i.e. correct to about one ulp of double
-  const double r00 = q(0, 0) + R00, r01 = q(1, 0) + R01, r11 = q(1, 1) + R11;
+  const double r00 = C0[0] + R00, r01 = C0[1] + R01, r11 = C1[1] + R11;
   const double det = r00 * r11 - r01 * r01;
   double s = __builtin_amdgcn_rcp(det);
   s = __builtin_fma(s, __builtin_fma(-det, s, 1.0), s);
@@ -629,8 +654,8 @@ __device__ inline void update_fp64gain(Track& t, const Hit& h, Trig* g = nullptr
   double K0[6], K1[6];
 #pragma unroll
   for (int a = 0; a < n; ++a) {
-    K0[a] = q(a, 0) * i00 + q(a, 1) * i01;
-    K1[a] = q(a, 0) * i01 + q(a, 1) * i11;
+    K0[a] = C0[a] * i00 + C1[a] * i01;
+    K1[a] = C0[a] * i01 + C1[a] * i11;
   }
   const double res0 = (double)h.x - (double)p[0];
   const double res1 = (double)h.y - (double)p[1];
@@ -656,22 +681,24 @@ __device__ inline void update_fp64gain(Track& t, const Hit& h, Trig* g = nullptr
     rotate(g->sinT, g->cosT, sd, omcd);
   }
   // Joseph form (I - K H) P (I - K H)^T + K R K^T, expanded:
-  //   P - K (H P) - (P H^T) K^T + K S K^T,   S = H P H^T + R,
-  // entry (a, b) = P_ab - K_a.(HP)_b + (K_a S - (P H^T)_a).K_b.
-  // D_a = K_a S - (P H^T)_a vanishes for the exact optimal gain; computing it
-  // keeps the form's insensitivity to errors in K.
-  // (two FMAs each, from -(P H^T)_a)
-  double D0[6], D1[6];
+  //   P - K C^T - C K^T + K S K^T,   C = P H^T,  S = H P H^T + R,
+  // grouped symmetrically as P + M K^T + K M^T with M = K S / 2 - C:
+  // entry (a, b) = P_ab + M_a.K_b + K_a.M_b, four FMAs.  K S - C vanishes
+  // for the exact optimal gain; keeping it (in M) keeps the form's
+  // insensitivity to errors in K.  Only K and M stay live over the 15
+  // entries (the entries of P are widened one at a time).
+  const double h00 = 0.5 * r00, h01 = 0.5 * r01, h11 = 0.5 * r11;  // S / 2, exact
+  double M0[6], M1[6];
 #pragma unroll
   for (int a = 0; a < n; ++a) {
-    D0[a] = __builtin_fma(K0[a], r00, __builtin_fma(K1[a], r01, -q(a, 0)));
-    D1[a] = __builtin_fma(K0[a], r01, __builtin_fma(K1[a], r11, -q(a, 1)));
+    M0[a] = __builtin_fma(K0[a], h00, __builtin_fma(K1[a], h01, -C0[a]));
+    M1[a] = __builtin_fma(K0[a], h01, __builtin_fma(K1[a], h11, -C1[a]));
   }
 #pragma unroll
   for (int a = 0; a < n; ++a)
 #pragma unroll
     for (int b = 0; b <= a; ++b) {
-      const double v = q(a, b) - K0[a] * q(0, b) - K1[a] * q(1, b) + D0[a] * K0[b] + D1[a] * K1[b];
+      const double v = q(a, b) + M0[a] * K0[b] + M1[a] * K1[b] + K0[a] * M0[b] + K1[a] * M1[b];
       P[S(L::at(a), L::at(b))] = (float)v;
     }
 }

@@ -72,6 +72,27 @@ constexpr uint32_t kInvPio4[24] = {
 
 __host__ __device__ inline uint32_t top12(uint32_t bits) { return (bits >> 20) & 0x7ffu; }
 
+// Payne-Hanek reduction of a finite float with |y| >= 120 (bits: its bit
+// pattern): r = the reduced argument, n = the quadrant (the low two bits of
+// round(y * 2/pi)).  Kept out of line: inline, its 64-bit products would add
+// their registers to every layer loop that calls sincosf_glibc, where real
+// arguments never reach it.
+struct LargeReduction {
+  double r;
+  int n;
+};
+__host__ __device__ __attribute__((noinline)) inline LargeReduction reduce_large(uint32_t bits) {
+  const uint32_t* w = &kInvPio4[(bits >> 26) & 15u];
+  const uint32_t m = ((bits & 0xffffffu) | 0x800000u) << ((bits >> 23) & 7u);
+  const uint64_t lo = (uint64_t)m * w[8];
+  const uint64_t mid = (uint64_t)m * w[4];
+  const uint64_t hi = (uint64_t)(uint32_t)(m * w[0]);  // only its low 32 bits matter
+  uint64_t f = ((lo >> 32) | (hi << 32)) + mid;
+  const uint64_t n = (f + (1ull << 61)) >> 62;
+  f -= n << 62;
+  return LargeReduction{(double)(int64_t)f * kPiOver2To63, (int)n};
+}
+
 }  // namespace glibc_sincosf
 
 // sin(y) and cos(y) with glibc's bits.
@@ -101,17 +122,10 @@ __host__ __device__ inline void sincosf_glibc(float y, float* sp, float* cp) {
     quad = n;
     sgn = n;
   } else {
-    const uint32_t* w = &kInvPio4[(bits >> 26) & 15u];
-    const uint32_t m = ((bits & 0xffffffu) | 0x800000u) << ((bits >> 23) & 7u);
-    const uint64_t lo = (uint64_t)m * w[8];
-    const uint64_t mid = (uint64_t)m * w[4];
-    const uint64_t hi = (uint64_t)(uint32_t)(m * w[0]);  // only its low 32 bits matter
-    uint64_t f = ((lo >> 32) | (hi << 32)) + mid;
-    const uint64_t n = (f + (1ull << 61)) >> 62;
-    f -= n << 62;
-    r = (double)(int64_t)f * kPiOver2To63;
-    quad = (int)n;
-    sgn = (int)n + (int)(bits >> 31);
+    const LargeReduction red = reduce_large(bits);
+    r = red.r;
+    quad = red.n;
+    sgn = red.n + (int)(bits >> 31);
   }
   // glibc's sign table {+1, -1, -1, +1}, indexed by sgn & 3, applied to r.
   const double rs = ((sgn + 1) & 2) ? -r : r;
