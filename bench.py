@@ -177,10 +177,33 @@ def cpu_baseline_port(nevts: int, ntrks: int, nlayer: int) -> dict | None:
             "host": host_cpu()}
 
 
+# The GPU's clocks rise over the first tens of milliseconds of sustained work
+# after an idle spell (measured on the headline pass: 0.33 ms for the first
+# launches, 0.12 ms after 30 ms, a steady 0.116 ms from about 40 ms on). Every
+# timed figure follows at least this much untimed work of the same kind, so
+# it measures the sustained rate, not the clock ramp.
+SETTLE_S = 0.25
+
+
+def settle(ctx, fn, seconds: float = SETTLE_S) -> int:
+    """Runs fn() back to back, untimed, for `seconds` of wall time; returns
+    the number of calls."""
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            fn()
+        n += 10
+        ctx.sync()
+    return n
+
+
 def time_ms(ctx, fn, reps: int) -> float:
-    """Average hipEvent time of fn() over reps back-to-back calls, after reps
-    untimed calls: the GPU idles while the host generates a workload, and
-    its clocks take a few launches to come back up."""
+    """Average hipEvent time of fn() over reps back-to-back calls, after the
+    settling run and reps untimed calls: the GPU idles while the host
+    generates a workload, and its clocks take tens of milliseconds of work to
+    come back up."""
+    settle(ctx, fn)
     for _ in range(reps):
         fn()
     ctx.sync()
@@ -450,7 +473,8 @@ def main() -> int:
     dt, dh, do = ctx.upload(trk), ctx.upload(hit), ctx.alloc(trk.nbytes)
     tl_per_step = nevts * nb * W * nl
 
-    log(f"warmup {a.warmup} steps")
+    log(f"settling the device ({SETTLE_S} s of untimed passes), then warmup {a.warmup} steps")
+    settle_passes = settle(ctx, lambda: ctx.run(dt, dh, do, nevts, nb, nl, W, a.math))
     for _ in range(a.warmup):
         ctx.run(dt, dh, do, nevts, nb, nl, W, a.math)
     ctx.sync()
@@ -546,6 +570,10 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "wall_s_timed": round(wall, 4),
+        "device_settle": {"seconds": SETTLE_S, "passes": settle_passes,
+                          "why": "untimed passes before the warmup steps: the GPU's clocks take tens of "
+                                 "milliseconds of sustained work to rise after the idle spell of the "
+                                 "host-side input generation"},
         "z_on_last_hit_plane": z_ok,
     }
     if cpu:
