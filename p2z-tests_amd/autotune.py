@@ -22,6 +22,7 @@ import argparse
 import itertools
 import json
 import os
+import time
 import sys
 
 import numpy as np
@@ -37,13 +38,24 @@ CLAIMS = (1, 2, 4, 8)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "p2z", "autotune.json")
 
 
+def _settle(ctx, fn, seconds: float = 0.25) -> None:
+    """Runs fn() back to back, untimed, for `seconds` of wall time: after the
+    GPU idles (the host generating a sample), its engine clock takes tens of
+    milliseconds of sustained work to come back up."""
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(4):
+            fn()
+        ctx.sync()
+
+
 def _time(ctx, fn, reps: int, rounds: int = 3) -> float:
     """Sustained time per launch: the median over `rounds` of the hipEvent
     time of `reps` back-to-back launches, divided by reps.  Back-to-back
-    launches keep the GPU under load, as in a production pass; single
-    launches after an idle gap run at higher clocks and rank the candidates
-    differently.  reps untimed launches come first: after the GPU idles (the
-    host generating a sample), its clocks take a few launches to settle."""
+    launches keep the GPU under load, as in a production pass, and rank the
+    candidates as that pass would.  reps untimed launches come first (the
+    sweep's candidates follow each other, so the clock stays up; a single
+    configuration is settled first, _settle)."""
     for _ in range(reps):
         fn()
     ctx.sync()
@@ -73,6 +85,12 @@ def tune(ctx, kind: str = "ragged", nevents: int = 200, math: str = "fp64gain", 
         lanes = ctx.upload(np.ascontiguousarray(w.lane_nlayer))
         blk = ctx.upload(np.ascontiguousarray(w.blocks_lpt))
         out = ctx.alloc(w.tracks.nbytes)
+        # the GPU idled while the host generated the sample: its clock comes
+        # back up under the first candidate before anything is timed
+        f0, tpw0, pf0, wpc0, cl0 = cands[0]
+        _settle(ctx, lambda: ctx.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, math,
+                                            persistent=f0 == "persistent", tracks_per_wg=tpw0, prefetch=pf0,
+                                            waves_per_cu=wpc0, items_per_claim=cl0))
         recs = []
         for form, tpw, pf, wpc, cl in cands:
             pers = form == "persistent"
@@ -143,9 +161,10 @@ def run_one(ctx, spec: str, kind: str, nevents: int, math: str, reps: int, seed:
     lanes = ctx.upload(np.ascontiguousarray(w.lane_nlayer))
     blk = ctx.upload(np.ascontiguousarray(w.blocks_lpt))
     out = ctx.alloc(w.tracks.nbytes)
-    ms = _time(ctx, lambda: ctx.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, math, persistent=form == "persistent",
-                                           graph=True, tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc,
-                                           items_per_claim=cl), reps)
+    fn = lambda: ctx.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, math, persistent=form == "persistent",
+                                graph=True, tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc, items_per_claim=cl)
+    _settle(ctx, fn)
+    ms = _time(ctx, fn, reps)
     return dict(form=form, width=W, tracks_per_wg=tpw, prefetch=pf, waves_per_cu=wpc, items_per_claim=cl, ms=ms,
                 tl_per_s=w.track_layers / (ms * 1e-3), track_layers=w.track_layers, tracks=w.ntracks)
 
