@@ -71,3 +71,44 @@ def test_precision_report(ctx):
     assert g["cov"]["median"] <= f["cov"]["median"]
     assert g["cov"]["p99"] <= f["cov"]["p99"]
     assert g["par"]["frac_le_1e-5"] >= f["par"]["frac_le_1e-5"]
+
+
+def test_ragged_precision_report(ctx):
+    """The 2000 uneven events (6.3e6 tracks, 1.1e8 track-layers) through the
+    batched persistent form (one graph-captured launch, W=32): per-track error
+    of FAST and of the FP64-gain path against the full-FP64 oracle, beside the
+    reference FP32 algorithm's own error.  The throughput of the same launch
+    is bench.py's ragged section."""
+    from test_gpu_ragged import by_track, oracle_ragged
+    W = 32
+    w = p2z.Workload("ragged", nevents=2000, width=W, seed=2024)
+    print(f"case ragged-precision/full-size tracks={w.ntracks} track-layers={w.track_layers}", flush=True)
+    exp = oracle_ragged(w, fp64=True)
+    o32 = oracle_ragged(w)
+    trk, hit = ctx.upload(w.tracks), ctx.upload(w.hits)
+    lanes = ctx.upload(np.ascontiguousarray(w.lane_nlayer))
+    blk = ctx.upload(np.ascontiguousarray(w.blocks_lpt))
+    out = ctx.alloc(w.tracks.nbytes)
+    rows = {"reference fp32 (float oracle)": by_track(w, o32)}
+    for math in ("fast", "fp64gain"):
+        ctx.run_ragged(trk, hit, out, lanes, blk, w.nblocks, W, math, persistent=True, graph=True)
+        rows[f"gpu {math} (persistent)"] = by_track(w, ctx.download(out))
+    for b_ in (trk, hit, lanes, blk, out):
+        b_.free()
+    res = {}
+    for name, a in rows.items():
+        par, cov, nonfin = per_track_errors(a, exp)
+        res[name] = {"par": quantiles(par), "cov": quantiles(cov), "nonfinite_fp64": nonfin,
+                     "nonfinite_here": int((~np.all(np.isfinite(a[:, :6]), axis=1)).sum())}
+    print("RAGGED_PRECISION_REPORT " + json.dumps({"tracks": int(w.ntracks), "track_layers": int(w.track_layers),
+                                                   "report": res}), flush=True)
+    base = res["reference fp32 (float oracle)"]
+    f, g = res["gpu fast (persistent)"], res["gpu fp64gain (persistent)"]
+    assert f["nonfinite_here"] <= base["nonfinite_here"] and g["nonfinite_here"] <= base["nonfinite_here"], res
+    # FAST: no less accurate than the reference FP32 algorithm at the median and p99
+    assert f["par"]["median"] <= 1.1 * base["par"]["median"] + 1e-12
+    assert f["par"]["p99"] <= 1.25 * base["par"]["p99"] + 1e-12
+    # the FP64 gain steps: more accurate than FAST, parameters within 1e-5 of FP64 at the median
+    assert g["par"]["median"] <= f["par"]["median"] * 1.01 + 1e-12
+    assert g["cov"]["median"] <= f["cov"]["median"]
+    assert g["par"]["median"] <= 1e-5
