@@ -23,8 +23,9 @@ struct LaunchCfg {
 };
 
 // Hit-staging depth of the fused pass when none is set: FAST runs 5 waves per
-// SIMD at depth 1 (94 VGPRs) against 4 at depth 2 (103) and measures about 1%
-// faster; the FP64-gain and reference modes run at 3-4 waves either way and
+// SIMD at depth 1 (89 VGPRs) against 4 at depth 2 (100) and measures about 1%
+// faster; the reference mode (92 / 100 VGPRs) and the FP64-gain mode (126 /
+// 138) measure the same at both depths or slightly faster at depth 2, and
 // keep depth 2.
 inline int default_hit_stage(int math) { return math == 0 ? 1 : 2; }
 

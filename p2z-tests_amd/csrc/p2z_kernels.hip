@@ -219,7 +219,7 @@ k_ragged_persistent(const float* __restrict__ trk, const float* __restrict__ hit
       // the lane index hidden from loop-invariant code motion, so that
       // lane-dependent values are not hoisted out of the item loop into
       // registers held across all of it (10 VGPRs in every mode: FAST then
-      // fits 96 VGPRs, 5 waves per SIMD, as the one-shot form does)
+      // fits within 96 VGPRs, 5 waves per SIMD, as the one-shot form does)
       asm volatile("" : "+v"(lane_i));
       ragged_item<W, kMath, kPrefetch>(trk, hit, out, lane_nlay, blocks, nblocks, item, lane_i);
     }
