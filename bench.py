@@ -336,7 +336,10 @@ def section_serving(ctx, pool: dict, rates, budget_gb: float, resident: bool) ->
                          "p50_ms": rep.lat_p50 * 1e3, "p99_ms": rep.lat_p99 * 1e3, "p999_ms": rep.lat_p999 * 1e3,
                          "mean_ms": rep.lat_mean * 1e3, "launches" if resident else "batches": int(rep.batches),
                          "busy_rejections": int(rep.busy_rejections),
-                         "max_inflight_events": int(rep.max_inflight_events)})
+                         "max_inflight_events": int(rep.max_inflight_events),
+                         "span_ms": rep.t_span * 1e3,
+                         "host_submit_us_per_event": rep.host_submit_s / rep.nevents * 1e6,
+                         "host_pump_us_per_event": rep.host_pump_s / rep.nevents * 1e6})
     return rows
 
 
