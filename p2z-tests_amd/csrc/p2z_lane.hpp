@@ -192,7 +192,9 @@ __device__ inline void update_ref(Track& t, const Hit& h) {
 // quadrant: true with a wide margin for |x| < 2^20 (k < 2^20; the product's
 // rounding error stays below 2^-32).  Larger |x| (never a track angle) take
 // the C-library-faithful sincosf_glibc, whose Payne-Hanek reduction is exact
-// at any magnitude; NaN and infinity give NaN.
+// at any magnitude; so do infinity and NaN, which give NaN there (the
+// reduction below never sees a NaN, whose conversion to an integer would be
+// undefined).
 // The library's sincosf is faithful only (measured: 10-25% of its results at
 // arguments like the track angles phi ~ 1.4 and theta ~ 2.8 are one ulp from
 // the nearest float); the tracks of an event share nearly the same angles, so
@@ -209,7 +211,7 @@ __device__ __forceinline__ double sgpr_const(double v) {
 }
 
 __device__ inline void sincos_accurate(float xf, float* sp, float* cp) {
-  if (__builtin_fabsf(xf) >= 0x1p20f) {
+  if (!(__builtin_fabsf(xf) < 0x1p20f)) {  // large, infinite or NaN
     sincosf_glibc(xf, sp, cp);
     return;
   }
