@@ -17,19 +17,31 @@
 
 namespace {
 
-enum Op { kFmaF32, kMulF32, kRcpF32, kFmaF64, kMulF64, kCvtMulCvt, kRcpF64, kNumOps };
+enum Op {
+  kFmaF32, kMulF32, kRcpF32, kFmaF64, kMulF64, kCvtMulCvt, kRcpF64, kAddF32, kShlXor, kCmpSel,
+  kRotU32, kFmaF32FmaF64, kFmaF32RcpF32, kFmaF32ShlXor, kFmaF32RotU32, kNumOps
+};
+// Operations of one chain step; the last three interleave two independent
+// chains of different instruction kinds, to show whether they share one issue
+// slot (cost = sum of the two) or overlap.
 const char* const kNames[kNumOps] = {"v_fma_f32", "v_mul_f32", "v_rcp_f32", "v_fma_f64", "v_mul_f64",
-                                     "cvt_f64_f32 + v_mul_f64 + cvt_f32_f64", "v_rcp_f64"};
+                                     "cvt_f64_f32 + v_mul_f64 + cvt_f32_f64", "v_rcp_f64", "v_add_f32",
+                                     "v_lshlrev_b32 + v_xor_b32", "v_add_f32 + v_cmp_gt_f32 + v_cndmask_b32",
+                                     "v_alignbit_b32", "v_fma_f32 | v_fma_f64", "v_fma_f32 | v_rcp_f32",
+                                     "v_fma_f32 | v_lshlrev_b32 + v_xor_b32", "v_fma_f32 | v_alignbit_b32"};
 
 template <int kOp>
 __global__ __launch_bounds__(256) void k_rate(float* out, float a, float b, int iters) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  float x[8];
+  float x[8], y[8];
   double d[8];
+  uint32_t u[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     x[i] = a + tid * 1e-7f + i;
+    y[i] = x[i] + 0.5f;
     d[i] = x[i];
+    u[i] = (uint32_t)tid * 2654435761u + i;
   }
   for (int it = 0; it < iters; ++it) {
 #pragma unroll
@@ -44,11 +56,31 @@ __global__ __launch_bounds__(256) void k_rate(float* out, float a, float b, int 
         x[i] = (float)d[i];
       }
       if constexpr (kOp == kRcpF64) d[i] = __builtin_amdgcn_rcp(d[i]);
+      if constexpr (kOp == kAddF32) x[i] = x[i] + a;
+      if constexpr (kOp == kShlXor) u[i] ^= u[i] << 1;
+      if constexpr (kOp == kCmpSel) x[i] = x[i] > b ? a : x[i] + b;
+      if constexpr (kOp == kRotU32) u[i] = (u[i] << 1) | (u[i] >> 31);
+      if constexpr (kOp == kFmaF32FmaF64) {
+        x[i] = __builtin_fmaf(x[i], a, b);
+        d[i] = __builtin_fma(d[i], (double)a, (double)b);
+      }
+      if constexpr (kOp == kFmaF32RcpF32) {
+        x[i] = __builtin_fmaf(x[i], a, b);
+        y[i] = __builtin_amdgcn_rcpf(y[i]);
+      }
+      if constexpr (kOp == kFmaF32ShlXor) {
+        x[i] = __builtin_fmaf(x[i], a, b);
+        u[i] ^= u[i] << 1;
+      }
+      if constexpr (kOp == kFmaF32RotU32) {
+        x[i] = __builtin_fmaf(x[i], a, b);
+        u[i] = (u[i] << 1) | (u[i] >> 31);
+      }
     }
   }
   float s = 0.f;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) s += x[i] + (float)d[i];
+  for (int i = 0; i < 8; ++i) s += x[i] + y[i] + (float)d[i] + (float)u[i];
   out[tid] = s;
 }
 
@@ -95,6 +127,14 @@ int main(int argc, char** argv) {
   rc |= time_op<kMulF64>(out, grid, iters, &ms[kMulF64]);
   rc |= time_op<kCvtMulCvt>(out, grid, iters, &ms[kCvtMulCvt]);
   rc |= time_op<kRcpF64>(out, grid, iters, &ms[kRcpF64]);
+  rc |= time_op<kAddF32>(out, grid, iters, &ms[kAddF32]);
+  rc |= time_op<kShlXor>(out, grid, iters, &ms[kShlXor]);
+  rc |= time_op<kCmpSel>(out, grid, iters, &ms[kCmpSel]);
+  rc |= time_op<kRotU32>(out, grid, iters, &ms[kRotU32]);
+  rc |= time_op<kFmaF32FmaF64>(out, grid, iters, &ms[kFmaF32FmaF64]);
+  rc |= time_op<kFmaF32RcpF32>(out, grid, iters, &ms[kFmaF32RcpF32]);
+  rc |= time_op<kFmaF32ShlXor>(out, grid, iters, &ms[kFmaF32ShlXor]);
+  rc |= time_op<kFmaF32RotU32>(out, grid, iters, &ms[kFmaF32RotU32]);
   if (rc) {
     std::fprintf(stderr, "launch failed\n");
     return 2;
