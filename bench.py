@@ -60,6 +60,7 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK = 256 * 4 * 16 * 2.4e9
 PROFILE_TRAFFIC = os.path.join(HERE, "profiles", "traffic_fused_w32_100ev.txt")
 PROFILE_VALU = os.path.join(HERE, "profiles", "valu_counters_fused_w32_100ev.md")
+PROFILE_STREAM = os.path.join(HERE, "profiles", "hbm_copy_bandwidth.txt")
 
 
 def log(msg: str) -> None:
@@ -90,6 +91,18 @@ def read_traffic(math: str):
     WRITE_SIZE, scaled by the copy-kernel calibration), if the committed
     profile summary for this workload and math mode exists."""
     return _profile_value(PROFILE_TRAFFIC, "traffic_bytes_per_launch", math)
+
+
+def achievable_hbm(achieved_gbs: float):
+    """The bandwidth a plain streaming kernel reaches on this GPU with the
+    main workload's read:write mix (5:1) and size (committed calibration),
+    and the fraction of it the pass reaches: the practical HBM ceiling beside
+    the datasheet peak."""
+    gbs = _profile_value(PROFILE_STREAM, "achievable_gbs", "main")
+    if gbs is None:
+        return None
+    return {"value": gbs, "unit": "GB/s", "frac": round(achieved_gbs / gbs, 4),
+            "kernel": "plain stream, 540 MiB read + 108 MiB written per launch, one float4 per thread"}
 
 
 def valu_roofline(math: str, tl_per_s: float):
@@ -456,6 +469,8 @@ def main() -> int:
     ap.add_argument("--nlayer", type=int, default=20)
     ap.add_argument("--bsize", type=int, default=32)
     ap.add_argument("--tracks-per-wg", type=int, default=0)
+    ap.add_argument("--hit-stage", type=int, default=0, choices=(0, 1, 2),
+                    help="layers of hits in flight ahead of the one computed (0: the library's default per mode)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sections", default="main,ref,config0,ragged,serving,ingest,precision",
                     help="comma list of main, ref, config0, ragged, serving, ingest, precision")
@@ -472,7 +487,7 @@ def main() -> int:
     nevts, ntrks, nl, W = a.nevts, a.ntrks, a.nlayer, a.bsize
     log(f"generating {nevts} events x {ntrks} tracks x {nl} layers (reference generator)")
     trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
-    ctx = p2z.Context(0, tracks_per_wg=a.tracks_per_wg)
+    ctx = p2z.Context(0, tracks_per_wg=a.tracks_per_wg, hit_stage=a.hit_stage)
     dt, dh, do = ctx.upload(trk), ctx.upload(hit), ctx.alloc(trk.nbytes)
     tl_per_step = nevts * nb * W * nl
 
@@ -504,7 +519,8 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": read_traffic(a.math) if default_workload else None,
                 "bytes_per_track_layer": b_tl, "launch_ms": round(ms_per_step, 5),
-                "valu": valu_roofline(a.math, value) if default_workload else None}
+                "valu": valu_roofline(a.math, value) if default_workload else None,
+                "achievable": achievable_hbm(achieved) if default_workload else None}
 
     extra = {}
     if "ref" in sections:
@@ -569,7 +585,7 @@ def main() -> int:
         "config": {"workload": f"propagateToZ+KalmanUpdate_v2 driver pass, {nevts} events x {ntrks} tracks x "
                                f"{nl} layers, AoSoA bsize {W}, fused single launch, inputs resident in HBM",
                    "nevts": nevts, "ntrks": ntrks, "nlayer": nl, "bsize": W, "math": a.math,
-                   "parallelism": "one GPU, one track per lane"},
+                   "hit_stage": a.hit_stage or "default", "parallelism": "one GPU, one track per lane"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "wall_s_timed": round(wall, 4),

@@ -19,22 +19,30 @@ namespace {
 
 enum Op {
   kFmaF32, kMulF32, kRcpF32, kFmaF64, kMulF64, kCvtMulCvt, kRcpF64, kAddF32, kShlXor, kCmpSel,
-  kRotU32, kFmaF32FmaF64, kFmaF32RcpF32, kFmaF32ShlXor, kFmaF32RotU32, kNumOps
+  kRotU32, kFmaF32FmaF64, kFmaF32RcpF32, kFmaF32ShlXor, kFmaF32RotU32, kFmaF32V, kMulF32V, kMulF32Lit, kFmaakF32, kFmaF64V, kMulF64V, kNumOps
 };
-// Operations of one chain step; the last three interleave two independent
-// chains of different instruction kinds, to show whether they share one issue
-// slot (cost = sum of the two) or overlap.
+// Operations of one chain step.  The plain FP32 ones take their loop-invariant
+// operands from scalar registers (a, b); the "VGPR operands only" ones from
+// per-lane values.  "x | y" interleaves two independent chains of different
+// instruction kinds, to show whether they share one issue slot (cost = sum of
+// the two) or overlap.
 const char* const kNames[kNumOps] = {"v_fma_f32", "v_mul_f32", "v_rcp_f32", "v_fma_f64", "v_mul_f64",
                                      "cvt_f64_f32 + v_mul_f64 + cvt_f32_f64", "v_rcp_f64", "v_add_f32",
                                      "v_lshlrev_b32 + v_xor_b32", "v_add_f32 + v_cmp_gt_f32 + v_cndmask_b32",
                                      "v_alignbit_b32", "v_fma_f32 | v_fma_f64", "v_fma_f32 | v_rcp_f32",
-                                     "v_fma_f32 | v_lshlrev_b32 + v_xor_b32", "v_fma_f32 | v_alignbit_b32"};
+                                     "v_fma_f32 | v_lshlrev_b32 + v_xor_b32", "v_fma_f32 | v_alignbit_b32",
+                                     "v_fma_f32, VGPR operands only", "v_mul_f32, VGPR operands only",
+                                     "v_mul_f32, literal operand", "v_fmaak_f32 (VGPRs + literal)",
+                                     "v_fma_f64, VGPR operands only", "v_mul_f64, VGPR operands only"};
 
 template <int kOp>
 __global__ __launch_bounds__(256) void k_rate(float* out, float a, float b, int iters) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   float x[8], y[8];
   double d[8];
+  // per-lane values near a and b that the compiler cannot fold into scalars
+  const float va = a + (float)(tid & 1) * 1e-7f, vb = b + (float)(tid & 2) * 1e-9f;
+  const double vda = va, vdb = vb;
   uint32_t u[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
@@ -72,6 +80,12 @@ __global__ __launch_bounds__(256) void k_rate(float* out, float a, float b, int 
         x[i] = __builtin_fmaf(x[i], a, b);
         u[i] ^= u[i] << 1;
       }
+      if constexpr (kOp == kFmaF32V) x[i] = __builtin_fmaf(x[i], va, vb);
+      if constexpr (kOp == kMulF32V) x[i] = x[i] * va;
+      if constexpr (kOp == kMulF32Lit) x[i] = x[i] * 0.99951171875f;
+      if constexpr (kOp == kFmaakF32) x[i] = __builtin_fmaf(x[i], va, 0.0009765625f);
+      if constexpr (kOp == kFmaF64V) d[i] = __builtin_fma(d[i], vda, vdb);
+      if constexpr (kOp == kMulF64V) d[i] = d[i] * vda;
       if constexpr (kOp == kFmaF32RotU32) {
         x[i] = __builtin_fmaf(x[i], a, b);
         u[i] = (u[i] << 1) | (u[i] >> 31);
@@ -135,6 +149,12 @@ int main(int argc, char** argv) {
   rc |= time_op<kFmaF32RcpF32>(out, grid, iters, &ms[kFmaF32RcpF32]);
   rc |= time_op<kFmaF32ShlXor>(out, grid, iters, &ms[kFmaF32ShlXor]);
   rc |= time_op<kFmaF32RotU32>(out, grid, iters, &ms[kFmaF32RotU32]);
+  rc |= time_op<kFmaF32V>(out, grid, iters, &ms[kFmaF32V]);
+  rc |= time_op<kMulF32V>(out, grid, iters, &ms[kMulF32V]);
+  rc |= time_op<kMulF32Lit>(out, grid, iters, &ms[kMulF32Lit]);
+  rc |= time_op<kFmaakF32>(out, grid, iters, &ms[kFmaakF32]);
+  rc |= time_op<kFmaF64V>(out, grid, iters, &ms[kFmaF64V]);
+  rc |= time_op<kMulF64V>(out, grid, iters, &ms[kMulF64V]);
   if (rc) {
     std::fprintf(stderr, "launch failed\n");
     return 2;
