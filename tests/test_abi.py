@@ -131,3 +131,25 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120, cwd=root)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and not r.stdout.strip()
+
+
+def test_bench_reads_the_committed_profile_figures():
+    """The roofline fields bench.py adds from the committed rocprofv3 and
+    calibration summaries parse, and stay consistent with each other: HBM
+    traffic within 1% of the algorithmic bytes of the main workload, every
+    mode's VALU count per track-layer present, and the achievable streaming
+    bandwidth between the copy kernel's range and the 8 TB/s peak."""
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    alg = 100 * 9600 * 20 * bench.bytes_per_track_layer(20)
+    for math in ("fast", "ref"):
+        traffic = bench.read_traffic(math)
+        assert traffic is not None and abs(traffic / alg - 1) < 0.01, (math, traffic, alg)
+    for math in p2z.MATH:
+        v = bench.valu_roofline(math, 1e11)
+        assert v is not None and 150 < v["lane_instr_per_tl"] < 1000, (math, v)
+    a = bench.achievable_hbm(5000.0)
+    assert a is not None and 4000 < a["value"] < bench.HBM_PEAK_GBS
+    assert abs(a["frac"] - 5000.0 / a["value"]) < 1e-3
