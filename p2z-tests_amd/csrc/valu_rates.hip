@@ -18,12 +18,14 @@
 namespace {
 
 enum Op {
-  kFmaF32, kMulF32, kRcpF32, kFmaF64, kMulF64, kCvtMulCvt, kRcpF64, kAddF32, kShlXor, kCmpSel,
-  kRotU32, kFmaF32FmaF64, kFmaF32RcpF32, kFmaF32ShlXor, kFmaF32RotU32, kFmaF32V, kMulF32V, kMulF32Lit, kFmaakF32, kFmaF64V, kMulF64V, kNumOps
+  kFmaF32, kMulF32, kRcpF32, kFmaF64, kMulF64, kCvtMulCvt, kRcpF64, kAddF32, kShlXor, kCmpSel, kRotU32,
+  kFmaF32FmaF64, kFmaF32RcpF32, kFmaF32ShlXor, kFmaF32RotU32,
+  kFmaF32V, kMulF32V, kMulF32Lit, kFmaakF32, kFmaF64V, kMulF64V, kFmaF32VFmaF64V, kNumOps
 };
 // Operations of one chain step.  The plain FP32 ones take their loop-invariant
 // operands from scalar registers (a, b); the "VGPR operands only" ones from
-// per-lane values.  "x | y" interleaves two independent chains of different
+// per-lane values (the compiler keeps the FP64 ones' constants in VGPRs
+// either way).  "x | y" interleaves two independent chains of different
 // instruction kinds, to show whether they share one issue slot (cost = sum of
 // the two) or overlap.
 const char* const kNames[kNumOps] = {"v_fma_f32", "v_mul_f32", "v_rcp_f32", "v_fma_f64", "v_mul_f64",
@@ -33,7 +35,8 @@ const char* const kNames[kNumOps] = {"v_fma_f32", "v_mul_f32", "v_rcp_f32", "v_f
                                      "v_fma_f32 | v_lshlrev_b32 + v_xor_b32", "v_fma_f32 | v_alignbit_b32",
                                      "v_fma_f32, VGPR operands only", "v_mul_f32, VGPR operands only",
                                      "v_mul_f32, literal operand", "v_fmaak_f32 (VGPRs + literal)",
-                                     "v_fma_f64, VGPR operands only", "v_mul_f64, VGPR operands only"};
+                                     "v_fma_f64, VGPR operands only", "v_mul_f64, VGPR operands only",
+                                     "v_fma_f32 | v_fma_f64, VGPR operands only"};
 
 template <int kOp>
 __global__ __launch_bounds__(256) void k_rate(float* out, float a, float b, int iters) {
@@ -86,6 +89,10 @@ __global__ __launch_bounds__(256) void k_rate(float* out, float a, float b, int 
       if constexpr (kOp == kFmaakF32) x[i] = __builtin_fmaf(x[i], va, 0.0009765625f);
       if constexpr (kOp == kFmaF64V) d[i] = __builtin_fma(d[i], vda, vdb);
       if constexpr (kOp == kMulF64V) d[i] = d[i] * vda;
+      if constexpr (kOp == kFmaF32VFmaF64V) {
+        x[i] = __builtin_fmaf(x[i], va, vb);
+        d[i] = __builtin_fma(d[i], vda, vdb);
+      }
       if constexpr (kOp == kFmaF32RotU32) {
         x[i] = __builtin_fmaf(x[i], a, b);
         u[i] = (u[i] << 1) | (u[i] >> 31);
@@ -155,6 +162,7 @@ int main(int argc, char** argv) {
   rc |= time_op<kFmaakF32>(out, grid, iters, &ms[kFmaakF32]);
   rc |= time_op<kFmaF64V>(out, grid, iters, &ms[kFmaF64V]);
   rc |= time_op<kMulF64V>(out, grid, iters, &ms[kMulF64V]);
+  rc |= time_op<kFmaF32VFmaF64V>(out, grid, iters, &ms[kFmaF32VFmaF64V]);
   if (rc) {
     std::fprintf(stderr, "launch failed\n");
     return 2;
