@@ -23,10 +23,10 @@ struct LaunchCfg {
 };
 
 // Hit-staging depth of the fused pass when none is set: FAST runs 5 waves per
-// SIMD at depth 1 (89 VGPRs) against 4 at depth 2 (100) and measures about 1%
+// SIMD at depth 1 (89 VGPRs) against 4 at depth 2 (100) and measures 1-3%
 // faster; the reference mode (92 / 100 VGPRs) and the FP64-gain mode (126 /
-// 138) measure the same at both depths or slightly faster at depth 2, and
-// keep depth 2.
+// 138) measure within 2% at the two depths, one way or the other from box to
+// box, and keep depth 2.
 inline int default_hit_stage(int math) { return math == 0 ? 1 : 2; }
 
 // Fused driver pass: ntracks = nblocks*W tracks, all with nlayer layers.
