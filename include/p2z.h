@@ -56,7 +56,7 @@ typedef struct p2z_ctx p2z_ctx;
 
 typedef struct {
   int32_t tracks_per_wg; /* workgroup size in tracks: 64, 128, 192 or 256; 0 = 256 */
-  int32_t nstreams;      /* streams of the host-buffer pipeline; 0 = 4            */
+  int32_t nstreams;      /* staging slots of the host-buffer pipeline; 0 = 4      */
   int32_t hit_stage;     /* hit-staging depth: layers of hits in flight ahead of
                             the one being computed, 1 or 2; 0 = per mode
                             (1 for P2Z_MATH_FAST, 2 otherwise)              */

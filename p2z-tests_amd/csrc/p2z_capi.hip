@@ -71,6 +71,22 @@ double* noalloc_scratch(int device) {
 namespace {
 using p2z::use_device;
 
+// Releases the compute streams, events and staging slots of the host-buffer
+// pipeline (the caller has waited for them).
+void free_pipeline_slots(p2z_ctx* c) {
+  for (auto s : c->pipe) (void)hipStreamDestroy(s);
+  for (auto* ev : {&c->ev_in, &c->ev_done, &c->ev_out}) {
+    for (auto e : *ev) (void)hipEventDestroy(e);
+    ev->clear();
+  }
+  for (auto p : c->slot_trk) (void)hipFree(p);
+  for (auto p : c->slot_hit) (void)hipFree(p);
+  for (auto p : c->slot_out) (void)hipFree(p);
+  c->pipe.clear();
+  c->slot_trk.clear(); c->slot_hit.clear(); c->slot_out.clear();
+  c->slot_trk_bytes = c->slot_hit_bytes = 0;
+}
+
 int apply_opts(p2z_ctx* c, const p2z_opts* o) {
   LaunchCfg cfg;
   int ns = 4;
@@ -91,15 +107,16 @@ int apply_opts(p2z_ctx* c, const p2z_opts* o) {
     if (o->hit_stage) cfg.hit_stage = o->hit_stage;
   }
   c->cfg = cfg;
+  if (!c->pipe_in) P2Z_HIP_CHECK(hipStreamCreateWithFlags(&c->pipe_in, hipStreamNonBlocking));
+  if (!c->pipe_out) P2Z_HIP_CHECK(hipStreamCreateWithFlags(&c->pipe_out, hipStreamNonBlocking));
   if ((int)c->pipe.size() != ns) {
-    for (auto s : c->pipe) (void)hipStreamDestroy(s);
-    for (auto p : c->slot_trk) (void)hipFree(p);
-    for (auto p : c->slot_hit) (void)hipFree(p);
-    for (auto p : c->slot_out) (void)hipFree(p);
-    c->slot_trk.clear(); c->slot_hit.clear(); c->slot_out.clear();
-    c->slot_trk_bytes = c->slot_hit_bytes = 0;
+    free_pipeline_slots(c);
     c->pipe.assign(ns, nullptr);
     for (auto& s : c->pipe) P2Z_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (auto* ev : {&c->ev_in, &c->ev_done, &c->ev_out}) {
+      ev->assign(ns, nullptr);
+      for (auto& e : *ev) P2Z_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
   }
   return P2Z_OK;
 }
@@ -166,11 +183,10 @@ int p2z_destroy(p2z_ctx* c) {
   if (!c) return P2Z_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto s : c->pipe) (void)hipStreamSynchronize(s);
-  for (auto p : c->slot_trk) (void)hipFree(p);
-  for (auto p : c->slot_hit) (void)hipFree(p);
-  for (auto p : c->slot_out) (void)hipFree(p);
-  for (auto s : c->pipe) (void)hipStreamDestroy(s);
+  (void)sync_pipeline(c);
+  free_pipeline_slots(c);
+  if (c->pipe_in) (void)hipStreamDestroy(c->pipe_in);
+  if (c->pipe_out) (void)hipStreamDestroy(c->pipe_out);
   if (c->sum_part) (void)hipFree(c->sum_part);
   if (c->sum_res) (void)hipFree(c->sum_res);
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -186,7 +202,7 @@ static int set_opts_impl(p2z_ctx* c, const p2z_opts* o) {
   if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
   P2Z_TRY(use_device(c));
   P2Z_HIP_CHECK(hipStreamSynchronize(c->stream));
-  for (auto s : c->pipe) P2Z_HIP_CHECK(hipStreamSynchronize(s));
+  P2Z_HIP_CHECK(sync_pipeline(c));
   return apply_opts(c, o);
 }
 
@@ -198,7 +214,7 @@ int p2z_sync(p2z_ctx* c) {
   if (!c) return p2z::fail(P2Z_EINVAL, "null ctx");
   P2Z_TRY(use_device(c));
   P2Z_HIP_CHECK(hipStreamSynchronize(c->stream));
-  for (auto s : c->pipe) P2Z_HIP_CHECK(hipStreamSynchronize(s));
+  P2Z_HIP_CHECK(sync_pipeline(c));
   return P2Z_OK;
 }
 
@@ -369,15 +385,22 @@ static int run_host_impl(p2z_ctx* c, const void* trk, const void* hit, void* out
   if (nevts == 0 || nb == 0) return P2Z_OK;
   const int ns = (int)c->pipe.size();
   const bool hit_pinned = nlayer > 0 && host_pinned(hit);
-  // Chunks of whole events, two per stream so that copies of one chunk overlap
-  // the compute of the previous one on the other streams.
-  const int64_t nchunks = std::min<int64_t>(nevts, 2 * (int64_t)ns);
+  // Chunks of whole events through ns staging slots, in three stages: every
+  // copy in runs on pipe_in, one after another at the full PCIe rate (copies
+  // in on several streams would share the link and reach their kernels, then
+  // their copies out, at the same time, leaving the link idle while they
+  // drain); chunk k's kernel runs on slot k % ns's stream; every copy out runs
+  // on pipe_out, beside the copies in.  Four chunks per slot keep the last
+  // kernel and copy out, which nothing overlaps, short.  A slot is reused once
+  // its previous kernel has read the inputs (ev_done) and its previous output
+  // has been copied out (ev_out).
+  const int64_t nchunks = std::min<int64_t>(nevts, 4 * (int64_t)ns);
   const int64_t per = (nevts + nchunks - 1) / nchunks;
   const size_t trk_ev = (size_t)nb * p2z::kTrkWords * width * sizeof(float);
   const size_t hit_ev = (size_t)nb * nlayer * p2z::kHitWords * width * sizeof(float);
   const size_t need_trk = trk_ev * per, need_hit = std::max<size_t>(hit_ev * per, 1);
   if (c->slot_trk_bytes < need_trk || c->slot_hit_bytes < need_hit || (int)c->slot_trk.size() != ns) {
-    for (auto s : c->pipe) P2Z_HIP_CHECK(hipStreamSynchronize(s));
+    P2Z_HIP_CHECK(sync_pipeline(c));
     for (auto p : c->slot_trk) (void)hipFree(p);
     for (auto p : c->slot_hit) (void)hipFree(p);
     for (auto p : c->slot_out) (void)hipFree(p);
@@ -400,10 +423,11 @@ static int run_host_impl(p2z_ctx* c, const void* trk, const void* hit, void* out
       const int64_t e0 = k * per, e1 = std::min(nevts, e0 + per);
       if (e0 >= e1) break;
       const int i = (int)(k % ns);
-      hipStream_t s = c->pipe[i];
+      const bool reuse = k >= ns;  // slot i held chunk k - ns
       const int64_t ne = e1 - e0;
+      if (reuse) P2Z_HIP_CHECK(hipStreamWaitEvent(c->pipe_in, c->ev_done[i], 0));
       P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_trk[i], (const char*)trk + e0 * trk_ev, ne * trk_ev,
-                                   hipMemcpyHostToDevice, s));
+                                   hipMemcpyHostToDevice, c->pipe_in));
       if (hit_pinned) {
         // only the words the path reads: pos x, y, z and cov xx, xy, yy of each
         // hit record (the record's last 3W words, cov xz, yz, zz, are never
@@ -411,24 +435,29 @@ static int run_host_impl(p2z_ctx* c, const void* trk, const void* hit, void* out
         const size_t pitch = (size_t)p2z::kHitWords * width * sizeof(float);
         P2Z_HIP_CHECK(hipMemcpy2DAsync(c->slot_hit[i], pitch, (const char*)hit + e0 * hit_ev, pitch,
                                        (size_t)p2z::kHitUsedWords * width * sizeof(float),
-                                       (size_t)(ne * nb * nlayer), hipMemcpyHostToDevice, s));
+                                       (size_t)(ne * nb * nlayer), hipMemcpyHostToDevice, c->pipe_in));
       } else {
         // pageable memory: one contiguous copy (a strided copy from pageable
         // memory goes row by row through the runtime's staging)
         P2Z_HIP_CHECK(hipMemcpyAsync(c->slot_hit[i], (const char*)hit + e0 * hit_ev, ne * hit_ev,
-                                     hipMemcpyHostToDevice, s));
+                                     hipMemcpyHostToDevice, c->pipe_in));
       }
+      P2Z_HIP_CHECK(hipEventRecord(c->ev_in[i], c->pipe_in));
+      hipStream_t s = c->pipe[i];
+      P2Z_HIP_CHECK(hipStreamWaitEvent(s, c->ev_in[i], 0));
+      if (reuse) P2Z_HIP_CHECK(hipStreamWaitEvent(s, c->ev_out[i], 0));
       P2Z_HIP_CHECK(p2z::launch_fused((const float*)c->slot_trk[i], (const float*)c->slot_hit[i],
                                       (float*)c->slot_out[i], ne * nb, nlayer, width, math, c->cfg, s));
+      P2Z_HIP_CHECK(hipEventRecord(c->ev_done[i], s));
+      P2Z_HIP_CHECK(hipStreamWaitEvent(c->pipe_out, c->ev_done[i], 0));
       P2Z_HIP_CHECK(hipMemcpyAsync((char*)out + e0 * trk_ev, c->slot_out[i], ne * trk_ev,
-                                   hipMemcpyDeviceToHost, s));
+                                   hipMemcpyDeviceToHost, c->pipe_out));
+      P2Z_HIP_CHECK(hipEventRecord(c->ev_out[i], c->pipe_out));
     }
     return P2Z_OK;
   }();
-  for (auto s : c->pipe) {
-    const hipError_t e = hipStreamSynchronize(s);
-    if (rc == P2Z_OK) P2Z_HIP_CHECK(e);
-  }
+  const hipError_t e = sync_pipeline(c);
+  if (rc == P2Z_OK) P2Z_HIP_CHECK(e);
   return rc;
 }
 

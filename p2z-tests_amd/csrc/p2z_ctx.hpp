@@ -11,10 +11,15 @@
 struct p2z_ctx {
   int device = 0;
   hipStream_t stream = nullptr;            // main stream: device-pointer entry points
-  std::vector<hipStream_t> pipe;           // host-buffer pipeline streams
+  // host-buffer pipeline: every copy in on pipe_in, every copy out on
+  // pipe_out, each staging slot's compute on its own stream in pipe; per slot,
+  // events mark its inputs copied, its kernel done and its output copied
+  std::vector<hipStream_t> pipe;
+  hipStream_t pipe_in = nullptr, pipe_out = nullptr;
+  std::vector<hipEvent_t> ev_in, ev_done, ev_out;
   hipEvent_t t0 = nullptr, t1 = nullptr;   // timer
   p2z::LaunchCfg cfg;
-  // host-buffer pipeline staging, one slot per pipeline stream
+  // host-buffer pipeline staging, one slot per compute stream
   std::vector<void*> slot_trk, slot_hit, slot_out;
   size_t slot_trk_bytes = 0, slot_hit_bytes = 0;
   // device report scratch
@@ -34,3 +39,17 @@ namespace p2z {
 // Makes the context's device current.
 int use_device(p2z_ctx* c);
 }  // namespace p2z
+
+// Waits for every stream of the host-buffer pipeline; the first error, if any.
+inline hipError_t sync_pipeline(p2z_ctx* c) {
+  hipError_t first = hipSuccess;
+  auto wait = [&](hipStream_t s) {
+    if (!s) return;
+    const hipError_t e = hipStreamSynchronize(s);
+    if (first == hipSuccess) first = e;
+  };
+  wait(c->pipe_in);
+  for (auto s : c->pipe) wait(s);
+  wait(c->pipe_out);
+  return first;
+}

@@ -385,6 +385,24 @@ def test_host_pipeline_equals_oracle(ctx, pinned):
     assert same_words(out, exp)
 
 
+@pytest.mark.parametrize("nstreams", [1, 3])
+def test_host_pipeline_slot_reuse_equals_oracle(nstreams):
+    """With one staging slot every chunk waits for the previous chunk's kernel
+    (before its copy in) and copy out (before its kernel); with three, chunks
+    of unequal size cycle through the slots.  Reference typing, word for word
+    against the oracle, twice in a row on the same context."""
+    nevts, ntrks, W, nl = 23, 1024, 32, 20
+    trk, hit, last, nb = p2z.generate("ref", nevts, ntrks, W, nl)
+    exp = O.run(trk, hit, nevts, nb, W, nl)
+    log(f"case run_host/slots nstreams={nstreams}")
+    with p2z.Context(0, nstreams=nstreams) as c:
+        ptrk, phit = c.pinned_copy(trk), c.pinned_copy(hit)
+        for rep in range(2):
+            pout = c.pinned_empty(trk.size)
+            c.run_host(ptrk, phit, pout, nevts, nb, nl, W, "ref")
+            assert same_words(np.array(pout), exp), rep
+
+
 def test_host_pipeline_equals_device_run(ctx):
     """The include-data pipeline (chunks over streams) computes the same bits."""
     nevts, ntrks, W, nl = 13, 1024, 32, 20
