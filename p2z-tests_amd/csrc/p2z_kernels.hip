@@ -26,13 +26,21 @@ __device__ inline void load_track(Track& t, const float* __restrict__ tb) {
   t.q = reinterpret_cast<const int32_t*>(tb)[27 * W];
 }
 
+// The output is written once and not read by the pass: non-temporal stores
+// keep it from displacing what the caches hold.  Measured on the 100-event
+// pass cycling through three buffer sets (nothing reused across passes from
+// the 256 MB last-level cache): 0.131 -> 0.123 ms; on the same buffers every
+// pass, where the track input is partly served from that cache, unchanged;
+// the 2000-event ragged pass 1-1.5% faster.  Non-temporal track loads as well
+// gave the same cold-cache time but lost that reuse (0.121 ms on the same
+// buffers).
 template <int W>
 __device__ inline void store_track(const Track& t, float* __restrict__ ob) {
 #pragma unroll
-  for (int e = 0; e < 6; ++e) ob[e * W] = t.p[e];
+  for (int e = 0; e < 6; ++e) __builtin_nontemporal_store(t.p[e], ob + e * W);
 #pragma unroll
-  for (int e = 0; e < 21; ++e) ob[(6 + e) * W] = t.P[e];
-  reinterpret_cast<int32_t*>(ob)[27 * W] = t.q;
+  for (int e = 0; e < 21; ++e) __builtin_nontemporal_store(t.P[e], ob + (6 + e) * W);
+  __builtin_nontemporal_store(t.q, reinterpret_cast<int32_t*>(ob) + 27 * W);
 }
 
 // Hits are read exactly once: non-temporal loads keep them out of the way of
