@@ -93,16 +93,19 @@ def read_traffic(math: str):
     return _profile_value(PROFILE_TRAFFIC, "traffic_bytes_per_launch", math)
 
 
-def achievable_hbm(achieved_gbs: float):
-    """The bandwidth a plain streaming kernel reaches on this GPU with the
-    main workload's read:write mix (5:1) and size (committed calibration),
-    and the fraction of it the pass reaches: the practical HBM ceiling beside
-    the datasheet peak."""
-    gbs = _profile_value(PROFILE_STREAM, "achievable_gbs", "main")
+def achievable_hbm(achieved_gbs: float, which: str = "main"):
+    """The bandwidth the best plain streaming kernel reaches on this GPU with
+    the main workload's read:write mix (5:1) and size (committed
+    calibration): on one buffer set ("main", as the timed passes run) or
+    cycling through three ("cold", no reuse from the last-level cache); and
+    the fraction of it the pass reaches.  The practical HBM ceiling beside the
+    datasheet peak."""
+    gbs = _profile_value(PROFILE_STREAM, "achievable_gbs", which)
     if gbs is None:
         return None
+    sets = "one buffer set" if which == "main" else "three buffer sets in turn"
     return {"value": gbs, "unit": "GB/s", "frac": round(achieved_gbs / gbs, 4),
-            "kernel": "plain stream, 540 MiB read + 108 MiB written per launch, one float4 per thread"}
+            "kernel": f"plain stream, 540 MiB read + 108 MiB written per launch, {sets}"}
 
 
 def valu_roofline(math: str, tl_per_s: float):
@@ -257,8 +260,9 @@ def cold_cache_pass(ctx, trk, hit, first, run, tl_per_step: int, b_tl: float) ->
         for b in bufs:
             b.free()
     tlps = tl_per_step / (ms * 1e-3)
+    gbs = tlps * b_tl / 1e9
     return {"buffer_sets": 3, "ms_per_step": round(ms, 5), "value": tlps,
-            "hbm_frac": round(tlps * b_tl / (HBM_PEAK_GBS * 1e9), 4)}
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "achievable": achievable_hbm(gbs, "cold")}
 
 
 def section_ragged(ctx, nevents: int, sample_events: int) -> tuple[dict, dict]:
