@@ -441,7 +441,7 @@ def section_ref_mode(ctx, trk, hit, nevts: int, nb: int, nl: int, W: int) -> dic
     tl = nevts * nb * W * nl
     tlps = tl / (ms * 1e-3)
     gbs = tlps * bytes_per_track_layer(nl) / 1e9
-    return {"kernel": f"k_fused<{W},REF,2>", "ms": round(ms, 5), "tl_per_s": tlps,
+    return {"kernel": f"k_fused<{W},REF,1>", "ms": round(ms, 5), "tl_per_s": tlps,
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "achieved_gb_s": round(gbs, 1),
             "traffic": read_traffic("ref"), "valu": valu_roofline("ref", tlps)}
 

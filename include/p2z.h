@@ -59,7 +59,7 @@ typedef struct {
   int32_t nstreams;      /* staging slots of the host-buffer pipeline; 0 = 4      */
   int32_t hit_stage;     /* hit-staging depth: layers of hits in flight ahead of
                             the one being computed, 1 or 2; 0 = per mode
-                            (1 for P2Z_MATH_FAST, 2 otherwise)              */
+                            (2 for P2Z_MATH_FP64GAIN, 1 otherwise)          */
   int32_t reserved;
 } p2z_opts;
 

@@ -24,10 +24,10 @@ struct LaunchCfg {
 
 // Hit-staging depth of the fused pass when none is set: FAST runs 5 waves per
 // SIMD at depth 1 (89 VGPRs) against 4 at depth 2 (100) and measures 1-3%
-// faster; the reference mode (92 / 100 VGPRs) and the FP64-gain mode (126 /
-// 138) measure within 2% at the two depths, one way or the other from box to
-// box, and keep depth 2.
-inline int default_hit_stage(int math) { return math == 0 ? 1 : 2; }
+// faster; the reference mode (92 / 100 VGPRs: 5 waves against 4) 1-3% faster
+// at depth 1 on three boxes; the FP64-gain mode (126 / 138 VGPRs: 4 waves
+// against 3) within 1% either way, and keeps depth 2.
+inline int default_hit_stage(int math) { return math == 2 ? 2 : 1; }
 
 // Fused driver pass: ntracks = nblocks*W tracks, all with nlayer layers.
 hipError_t launch_fused(const float* trk, const float* hit, float* out, int64_t nblocks,
